@@ -1,0 +1,157 @@
+"""ctypes binding of libbitgen_hip.so (declared in include/bitgen_hip.h).
+
+The product path has no fallback: if the library is missing or fails to
+load, every entry point raises ``HipRuntimeError``.
+"""
+import ctypes
+import os
+import threading
+
+from ...csrc import build as native_build
+
+
+class HipRuntimeError(RuntimeError):
+    pass
+
+
+class GroupArgs(ctypes.Structure):
+    """Mirror of ``bg_group_args_t`` (field order is ABI)."""
+
+    _fields_ = [
+        ("basis", ctypes.c_void_p),
+        ("result", ctypes.c_void_p),
+        ("status", ctypes.c_void_p),
+        ("counter", ctypes.c_void_p),
+        ("n_words", ctypes.c_uint64),
+        ("n_tiles", ctypes.c_uint32),
+        ("tile_base", ctypes.c_uint32),
+        ("epoch", ctypes.c_uint32),
+        ("n_carry", ctypes.c_uint32),
+    ]
+
+
+class KernelInfo(ctypes.Structure):
+    _fields_ = [
+        ("vgprs", ctypes.c_int32),
+        ("sgprs", ctypes.c_int32),
+        ("lds_bytes", ctypes.c_int32),
+        ("scratch_bytes", ctypes.c_int32),
+        ("max_threads", ctypes.c_int32),
+    ]
+
+
+EXPORTS = (
+    "bg_init", "bg_last_error", "bg_transpose", "bg_module_load", "bg_module_load_data",
+    "bg_module_unload", "bg_kernel_get", "bg_kernel_info", "bg_group_launch",
+    "bg_to_reference_layout", "bg_popcount", "bg_stream_sync",
+)
+
+_lib = None
+_lock = threading.Lock()
+_initialised = set()
+
+
+def library_path():
+    return os.environ.get("BITGEN_HIP_LIB", native_build.LIB_PATH)
+
+
+def load_library():
+    """Load the C ABI (no GPU call is made)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        path = library_path()
+        if not os.path.exists(path):
+            raise HipRuntimeError(
+                f"{path} is missing: build the hip backend first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(path)
+        vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+        L.bg_init.argtypes = [i32]
+        L.bg_last_error.restype = ctypes.c_char_p
+        L.bg_transpose.argtypes = [vp, u64, u64, vp, u64, u32, vp]
+        L.bg_module_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.bg_module_load_data.argtypes = [vp, ctypes.POINTER(vp)]
+        L.bg_module_unload.argtypes = [vp]
+        L.bg_kernel_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.bg_kernel_info.argtypes = [vp, ctypes.POINTER(KernelInfo)]
+        L.bg_group_launch.argtypes = [vp, ctypes.POINTER(GroupArgs), ctypes.POINTER(u32), vp]
+        L.bg_to_reference_layout.argtypes = [vp, vp, u64, u64, vp]
+        L.bg_popcount.argtypes = [vp, u64, vp, vp]
+        L.bg_stream_sync.argtypes = [vp]
+        for name in EXPORTS:
+            if name != "bg_last_error":
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+        return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load_library().bg_last_error().decode(errors="replace")
+        raise HipRuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def init(device=0):
+    L = load_library()
+    if device not in _initialised:
+        _check(L.bg_init(device), "bg_init")
+        _initialised.add(device)
+
+
+def current_stream_ptr(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def transpose(bytes_ptr, n_bytes, in_stride, basis_ptr, n_words, n_inputs, stream):
+    _check(load_library().bg_transpose(bytes_ptr, n_bytes, in_stride, basis_ptr, n_words, n_inputs, stream),
+           "bg_transpose")
+
+
+class Module:
+    def __init__(self, path):
+        self.path = path
+        h = ctypes.c_void_p()
+        _check(load_library().bg_module_load(path.encode(), ctypes.byref(h)), f"bg_module_load({path})")
+        self.handle = h
+        self.kernels = {}
+
+    def kernel(self, name):
+        if name not in self.kernels:
+            k = ctypes.c_void_p()
+            _check(load_library().bg_kernel_get(self.handle, name.encode(), ctypes.byref(k)), f"bg_kernel_get({name})")
+            self.kernels[name] = k
+        return self.kernels[name]
+
+    def info(self, name):
+        inf = KernelInfo()
+        _check(load_library().bg_kernel_info(self.kernel(name), ctypes.byref(inf)), "bg_kernel_info")
+        return {f: getattr(inf, f) for f, _ in KernelInfo._fields_}
+
+    def free(self):
+        if self.handle:
+            _check(load_library().bg_module_unload(self.handle), "bg_module_unload")
+            self.handle = None
+
+
+def group_launch(kernel, args, stream):
+    waves = ctypes.c_uint32()
+    _check(load_library().bg_group_launch(kernel, ctypes.byref(args), ctypes.byref(waves), stream),
+           "bg_group_launch")
+    return waves.value
+
+
+def to_reference_layout(src_ptr, dst_ptr, n_words, n_streams, stream):
+    _check(load_library().bg_to_reference_layout(src_ptr, dst_ptr, n_words, n_streams, stream),
+           "bg_to_reference_layout")
+
+
+def popcount(src_ptr, n_words, count_ptr, stream):
+    _check(load_library().bg_popcount(src_ptr, n_words, count_ptr, stream), "bg_popcount")
+
+
+def stream_sync(stream):
+    _check(load_library().bg_stream_sync(stream), "bg_stream_sync")

@@ -1,0 +1,152 @@
+"""Generated-module contract of the hip backend.
+
+Same contract as the reference's ``bitgen/backend/cuda/template/template.py``
+(``init_kernel`` at :124-149): ``init_kernel(regex_func_num,
+input_stream_tensor, basic_stream_tensor, cfg)`` returns ``(result,
+tmp_streams, kernel_handle, run, dyn_stats)`` where
+
+* ``result``: torch.uint64 ``[1, M, G, W]`` match-end streams, LSB-first words
+  (``to_reference_layout`` gives the reference's u32 ``[1, M, G, 2W]``);
+* ``tmp_streams``: the carry look-back status words ``[M, G, T*C]`` (uint32
+  stored as int32) -- the fused hip kernels spill no streams;
+* ``kernel_handle``: a ``HipGroupSet`` (``.free()`` unloads the modules);
+* ``run(basic_stream_tensor)``: launches every (input, group) pair on the
+  current torch stream and synchronises it, like CUDA_Kernel.run
+  (cuda_kernel.py:411-427).  Stores are plain (not ``|=``), so the caller
+  need not zero ``result`` first;
+* ``dyn_stats``: None.
+"""
+import os
+
+from . import runtime as rt
+
+WAVES_PER_BLOCK = 4
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+class HipGroupSet:
+    """Loaded group kernels of one pattern set plus their launch state."""
+
+    def __init__(self, metas, base_dir, device=0):
+        torch = _torch()
+        rt.init(device)
+        self.device = device
+        self.metas = metas
+        self.modules = []
+        self.kernels = []
+        for m in metas:
+            path = m["hsaco"] if os.path.isabs(m["hsaco"]) else os.path.join(base_dir, m["hsaco"])
+            mod = rt.Module(path)
+            self.modules.append(mod)
+            self.kernels.append(mod.kernel(m["kernel"]))
+        dev = torch.device("cuda", device)
+        # per group: [monotonic tile counter, look-back error flag] (launches of
+        # a group are ordered on one stream; groups never share a counter)
+        self.counters = torch.zeros((len(metas), 2), dtype=torch.int32, device=dev)
+        self.tile_base = [0] * len(metas)
+        self.epochs = {}
+
+    @property
+    def n_groups(self):
+        return len(self.metas)
+
+    @staticmethod
+    def n_tiles(meta, n_words):
+        s0 = 64 - meta["HR"]
+        if n_words <= s0:
+            return 1
+        return 1 + (n_words - s0 + meta["S"] - 1) // meta["S"]
+
+    def status_words(self, n_words):
+        return max(self.n_tiles(m, n_words) * max(1, m["n_carry"]) for m in self.metas)
+
+    def alloc_status(self, n_inputs, n_words):
+        torch = _torch()
+        return torch.zeros((n_inputs, self.n_groups, self.status_words(n_words)), dtype=torch.int32,
+                           device=torch.device("cuda", self.device))
+
+    def launch(self, g, basis_ptr, result_ptr, status_ptr, n_words, stream, status_key=None):
+        """Enqueue group g over one buffer (asynchronous)."""
+        meta = self.metas[g]
+        key = (g, status_key if status_key is not None else status_ptr)
+        epoch = self.epochs.get(key, 0) + 1
+        if epoch >= (1 << 28):
+            epoch = 1
+        self.epochs[key] = epoch
+        args = rt.GroupArgs()
+        args.basis = basis_ptr
+        args.result = result_ptr
+        args.status = status_ptr
+        args.counter = self.counters.data_ptr() + 8 * g
+        args.n_words = n_words
+        args.n_tiles = self.n_tiles(meta, n_words)
+        args.tile_base = self.tile_base[g] & 0xFFFFFFFF
+        args.epoch = epoch
+        args.n_carry = max(1, meta["n_carry"])
+        waves = rt.group_launch(self.kernels[g], args, stream)
+        self.tile_base[g] = (self.tile_base[g] + waves) & 0xFFFFFFFF
+        return waves
+
+    def run(self, basis, result, status, stream=None):
+        """basis u64 [S=1, M, 8, W]; result u64 [1, M, G, W]; status int32 [M, G, *]."""
+        torch = _torch()
+        if basis.dim() != 4 or basis.dtype != torch.uint64 or basis.size(2) != 8:
+            raise ValueError(f"basis must be uint64 [1, M, 8, W], got {tuple(basis.shape)} {basis.dtype}")
+        if basis.size(0) != 1:
+            raise NotImplementedError("split inputs (dimension 0 > 1) are not supported")
+        M, W = basis.size(1), basis.size(3)
+        if tuple(result.shape) != (1, M, self.n_groups, W) or result.dtype != torch.uint64:
+            raise ValueError(f"result must be uint64 [1, {M}, {self.n_groups}, {W}]")
+        if not (basis.is_contiguous() and result.is_contiguous() and status.is_contiguous()):
+            raise ValueError("tensors must be contiguous")
+        s = rt.current_stream_ptr(stream)
+        for m in range(M):
+            bptr = basis.data_ptr() + m * 8 * W * 8
+            for g in range(self.n_groups):
+                rptr = result.data_ptr() + (m * self.n_groups + g) * W * 8
+                sptr = status.data_ptr() + (m * self.n_groups + g) * status.size(2) * 4
+                self.launch(g, bptr, rptr, sptr, W, s)
+
+    def check_errors(self):
+        """Raise if any launch's carry look-back gave up (call after a sync)."""
+        flags = self.counters[:, 1].cpu()
+        if int(flags.abs().sum()) != 0:
+            bad = [g for g in range(self.n_groups) if int(flags[g]) != 0]
+            raise rt.HipRuntimeError(f"carry look-back timed out in groups {bad}; results are invalid")
+
+    def info(self):
+        return [mod.info(m["kernel"]) for mod, m in zip(self.modules, self.metas)]
+
+    def free(self):
+        for mod in self.modules:
+            mod.free()
+        self.modules = []
+        self.kernels = []
+
+
+def init_kernel(groups_meta, base_dir, regex_func_num, input_stream_tensor, basic_stream_tensor=None, cfg={}):
+    torch = _torch()
+    if basic_stream_tensor is None:
+        raise ValueError("the hip backend needs the transposed basis tensor")
+    basis = basic_stream_tensor
+    if basis.dim() != 4 or basis.dtype != torch.uint64:
+        raise ValueError("basic_stream_tensor must be uint64 [1, M, 8, W]")
+    device = basis.device.index if basis.device.index is not None else 0
+    ks = HipGroupSet(groups_meta, base_dir, device)
+    if regex_func_num != ks.n_groups:
+        raise ValueError(f"regex_func_num {regex_func_num} != {ks.n_groups} generated groups")
+    M, W = basis.size(1), basis.size(3)
+    result = torch.zeros((1, M, ks.n_groups, W), dtype=torch.uint64, device=basis.device)
+    status = ks.alloc_status(M, W)
+
+    def run(basic_stream_tensor):
+        ks.run(basic_stream_tensor, result, status)
+        rt.stream_sync(rt.current_stream_ptr())
+        ks.check_errors()
+
+    return result, status, ks, run, None

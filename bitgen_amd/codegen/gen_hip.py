@@ -1,0 +1,298 @@
+"""HIP (gfx950) code generator for the bitstream IR.
+
+Drop-in for the reference's ``CudaGenerator`` (``bitgen/codegen/gen_cuda.py:15-497``):
+same constructor ``(insts_list, var_name_map_list, regex_list)`` and
+``lower(input_path=None) -> path of kernels/kernel_0.py``.  It writes, under
+``cfg["__cached_module_path"]/kernels/``:
+
+* ``kernel_<g>.hip`` -- one ``extern "C" __global__ bg_group_<g>`` per regex
+  group (the reference emits one ``__device__ regex_<g>`` per group and an
+  if-chain dispatcher on blockIdx.x, gen_cuda.py:335-473);
+* ``kernel_<g>.hsaco`` -- the gfx950 code object (``hipcc --genco``, compiled
+  in parallel and cached by content hash);
+* ``kernel_0.py`` + ``__init__.py`` -- the generated-module contract
+  (``init_kernel`` returning ``(result, tmp_streams, kernel_handle, run,
+  dyn_stats)``, as bitgen/backend/cuda/template/template.py:124-149).
+
+Execution model of a group kernel (see template/hip/bitgen_device.h): a
+wavefront is a tile of 64 words; lane l holds word w0 + l of every stream.
+The generator sizes the halo from the program: ``need[v] = (back, ahead)`` is
+how many positions before/after the tile's output range v must be exact,
+propagated backwards from the store (Advance by n adds n to ``back`` of its
+operand, a look-ahead by n adds n to ``ahead``).  HL = ceil(max back / 64),
+HR = ceil(max ahead / 64) halo words; every carry operation k gets L_k, the
+first lane where its result must be exact, which is where the look-back
+carry enters.
+"""
+import concurrent.futures
+import json
+import math
+import os
+
+from .gen import Generator
+from ..inst import BsInstType
+from .. import config as cfg
+from ..csrc import build as native_build
+
+WAVE = 64
+
+
+class HipCodegenError(NotImplementedError):
+    pass
+
+
+def _cname(v):
+    return f"v{v}"
+
+
+class GroupProgram:
+    """Analysis + HIP source of one group program."""
+
+    def __init__(self, gid, insts, vnm, regexes=None):
+        self.gid = gid
+        self.insts = insts
+        self.vnm = vnm
+        self.regexes = regexes or []
+        self.kernel_name = f"bg_group_{gid}"
+        self._check()
+        self._halo()
+
+    def _check(self):
+        stores = 0
+        for inst in self.insts:
+            t = inst.type
+            if t in (BsInstType.WHILE, BsInstType.IF):
+                raise HipCodegenError(
+                    f"group {self.gid}: {t.name.lower()} blocks are not supported by the hip backend "
+                    "(star loops are rewritten to MatchStar; other loops have no tile-parallel form)")
+            if t in (BsInstType.SEL, BsInstType.STR, BsInstType.ADD):
+                raise HipCodegenError(f"group {self.gid}: unsupported instruction {inst}")
+            if t == BsInstType.STREAMSTORE:
+                stores += 1
+        if stores != 1:
+            raise HipCodegenError(f"group {self.gid}: expected exactly one stream store, found {stores}")
+
+    def _const(self, v):
+        n = self.vnm.get(v)
+        return n if n in ("0", "1") else None
+
+    def _halo(self):
+        need = {}
+
+        def bump(v, back, ahead):
+            if self._const(v) is not None:
+                return
+            b, a = need.get(v, (0, 0))
+            need[v] = (max(b, back), max(a, ahead))
+
+        for inst in reversed(self.insts):
+            t = inst.type
+            if t == BsInstType.STREAMSTORE:
+                bump(inst.operands[0], 0, 0)
+                continue
+            if t in (BsInstType.LABEL, BsInstType.GRAPHBREAK):
+                continue
+            if t == BsInstType.SKIPZERO:
+                # the zero test is exact whatever the halo holds
+                continue
+            r = need.get(inst.ret, (0, 0)) if inst.ret is not None else (0, 0)
+            if t == BsInstType.ADVANCE:
+                n = inst.amount
+                if inst.is_left:
+                    bump(inst.operands[0], max(0, r[0] - n), r[1] + n)
+                else:
+                    bump(inst.operands[0], r[0] + n, max(0, r[1] - n))
+            else:
+                for o in inst.uses():
+                    bump(o, r[0], r[1])
+        self.need = need
+        max_back = max((b for b, _ in need.values()), default=0)
+        max_ahead = max((a for _, a in need.values()), default=0)
+        self.HL = math.ceil(max_back / WAVE)
+        self.HR = math.ceil(max_ahead / WAVE)
+        self.S = WAVE - self.HL - self.HR
+        if self.S < 1:
+            raise HipCodegenError(
+                f"group {self.gid}: shift distances need {self.HL}+{self.HR} halo words, more than a 64-word tile")
+        self.carry_ops = []
+        for inst in self.insts:
+            if inst.type in (BsInstType.MATCHSTAR, BsInstType.SCANTHRU):
+                back = need.get(inst.ret, (0, 0))[0]
+                L = self.HL - math.ceil(back / WAVE)
+                self.carry_ops.append((inst, L))
+        self.n_carry = len(self.carry_ops)
+
+    def n_tiles(self, n_words):
+        s0 = WAVE - self.HR
+        if n_words <= s0:
+            return 1
+        return 1 + (n_words - s0 + self.S - 1) // self.S
+
+    def val(self, v):
+        c = self._const(v)
+        if c == "0":
+            return "0ull"
+        if c == "1":
+            return "~0ull"
+        return _cname(v)
+
+    def source(self):
+        lines = []
+        add = lines.append
+        regex_note = json.dumps(self.regexes)[:2000].replace("*/", "* /")
+        add('#include "bitgen_device.h"')
+        add(f"/* group {self.gid}: {regex_note} */")
+        add(f'extern "C" __global__ __launch_bounds__(256) void {self.kernel_name}(bg_group_args_t a) {{')
+        add(f"  constexpr int HL = {self.HL}, HR = {self.HR}, S = {self.S};")
+        add("  const uint32_t t = bg_acquire_tile(a.counter, a.tile_base);")
+        add("  if (t >= a.n_tiles) return;")
+        add("  const int lane = bg_lane();")
+        add("  const int64_t W = (int64_t)a.n_words;")
+        add("  const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;")
+        add("  const int64_t w = w0 + lane;")
+        add("  const bool inb = (w >= 0) && (w < W);")
+        add("  const bg_tile tc{t, w0, W, a.status, a.n_carry, a.epoch, a.counter + 1};")
+        add("  (void)tc;")
+        add("  uint64_t bas[8];")
+        add("#pragma unroll")
+        add("  for (int i = 0; i < 8; i++) bas[i] = inb ? a.basis[(int64_t)i * W + w] : 0ull;")
+        # declarations (goto must not cross initialisations)
+        vars_ = set()
+        for inst in self.insts:
+            for d in inst.defs():
+                vars_.add(d)
+            for u in inst.uses():
+                if self._const(u) is None:
+                    vars_.add(u)
+        vars_ = sorted(vars_)
+        for i in range(0, len(vars_), 16):
+            chunk = ", ".join(f"{_cname(v)} = 0" for v in vars_[i:i + 16])
+            add(f"  uint64_t {chunk};")
+        add("  uint64_t out = 0;")
+        carry_index = {id(inst): (k, L) for k, (inst, L) in enumerate(self.carry_ops)}
+        for inst in self.insts:
+            t = inst.type
+            if t == BsInstType.STREAMLOAD:
+                add(f"  {_cname(inst.ret)} = bas[{int(inst.stream_index)}];")
+            elif t == BsInstType.AND:
+                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])} & {self.val(inst.operands[1])};")
+            elif t == BsInstType.OR:
+                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])} | {self.val(inst.operands[1])};")
+            elif t == BsInstType.XOR:
+                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])} ^ {self.val(inst.operands[1])};")
+            elif t == BsInstType.NOT:
+                add(f"  {_cname(inst.ret)} = ~{self.val(inst.operands[0])};")
+            elif t == BsInstType.ASSIGN:
+                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])};")
+            elif t == BsInstType.ADVANCE:
+                fn = "bg_advl" if inst.is_left else "bg_advr"
+                add(f"  {_cname(inst.ret)} = {fn}<{inst.amount}>({self.val(inst.operands[0])});")
+            elif t in (BsInstType.MATCHSTAR, BsInstType.SCANTHRU):
+                k, L = carry_index[id(inst)]
+                fn = "bg_matchstar" if t == BsInstType.MATCHSTAR else "bg_scanthru"
+                add(f"  {_cname(inst.ret)} = {fn}<{L}, S, {k}>({self.val(inst.operands[0])}, "
+                    f"{self.val(inst.operands[1])}, tc);")
+            elif t == BsInstType.SKIPZERO:
+                add(f"  if (bg_wave_zero({self.val(inst.operands[0])})) goto L{inst.label};")
+            elif t == BsInstType.LABEL:
+                add(f"L{inst.label}:;")
+            elif t == BsInstType.STREAMSTORE:
+                add(f"  out = {self.val(inst.operands[0])};")
+            elif t == BsInstType.GRAPHBREAK:
+                pass
+            else:
+                raise HipCodegenError(f"unsupported instruction {inst}")
+        add("  const int lo = (t == 0) ? 0 : HL;")
+        add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
+        add("  if (inb && lane >= lo && lane < hi) a.result[w] = out;")
+        add("}")
+        return "\n".join(lines) + "\n"
+
+    def meta(self):
+        return {
+            "gid": self.gid,
+            "kernel": self.kernel_name,
+            "HL": self.HL,
+            "HR": self.HR,
+            "S": self.S,
+            "n_carry": self.n_carry,
+            "n_insts": len(self.insts),
+            "regexes": list(self.regexes),
+        }
+
+
+def _hsaco_cache_dir():
+    d = cfg.get_config("hsaco_cache_dir") or os.path.join(cfg.get_config("cache_dir", ".cache_bitgen"), "hsaco")
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def compile_sources(sources, workers=None):
+    """Compile {path: source} to cached hsaco files; returns {path: hsaco}."""
+    cache = _hsaco_cache_dir()
+    jobs = {}
+    out = {}
+    for path, src in sources.items():
+        key = native_build.source_key(src)
+        hs = os.path.join(cache, f"{key}.hsaco")
+        out[path] = hs
+        if not os.path.exists(hs):
+            jobs[path] = hs
+    if jobs:
+        workers = workers or min(len(jobs), int(os.environ.get("BITGEN_COMPILE_JOBS", "0")) or os.cpu_count() or 4, 16)
+        with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+            futs = [ex.submit(native_build.compile_kernel, p, hs) for p, hs in jobs.items()]
+            for f in futs:
+                f.result()
+    return out
+
+
+class HipGenerator(Generator):
+    def __init__(self, insts_list, var_name_map_list, regex_list=None):
+        super().__init__(insts_list, var_name_map_list, regex_list)
+        self.programs = []
+
+    def build_programs(self):
+        self.programs = []
+        for g, (insts, vnm) in enumerate(zip(self.insts_list, self.var_name_map_list)):
+            regs = self.regex_list[g] if self.regex_list else None
+            if isinstance(regs, str):
+                regs = [regs]
+            self.programs.append(GroupProgram(g, insts, vnm, regs))
+        return self.programs
+
+    def lower(self, input_path=None):
+        wrapper = self.get_kernel_path("kernel_0.py")
+        if cfg.get_config("use_cached_code") and os.path.exists(wrapper):
+            print(f"Use cached code: {wrapper}")
+            return wrapper
+        self.build_programs()
+        sources = {}
+        for prog in self.programs:
+            path = self.get_kernel_path(f"kernel_{prog.gid}.hip")
+            src = prog.source()
+            with open(path, "w") as f:
+                f.write(src)
+            sources[path] = src
+        hsacos = compile_sources(sources)
+        metas = []
+        for prog in self.programs:
+            m = prog.meta()
+            src_path = self.get_kernel_path(f"kernel_{prog.gid}.hip")
+            local = self.get_kernel_path(f"kernel_{prog.gid}.hsaco")
+            with open(hsacos[src_path], "rb") as fi, open(local, "wb") as fo:
+                fo.write(fi.read())
+            m["hsaco"] = os.path.basename(local)
+            metas.append(m)
+        with open(wrapper, "w") as f:
+            f.write('"""Generated by bitgen_amd.codegen.gen_hip (hip backend module)."""\n')
+            f.write("import os\n")
+            f.write("from bitgen_amd.backend.hip.template import init_kernel as _init_kernel\n\n")
+            f.write(f"GROUPS = {json.dumps(metas, indent=1)}\n")
+            f.write("HERE = os.path.dirname(os.path.abspath(__file__))\n")
+            f.write(f"grid_size = ({len(metas)}, {cfg.get_config('multi_input', 1)}, 1)\n")
+            f.write(f"block_size = {tuple(cfg.get_config('block_size', (256, 1, 1)))}\n\n\n")
+            f.write("def init_kernel(regex_func_num, input_stream_tensor, basic_stream_tensor=None, cfg={}):\n")
+            f.write("    return _init_kernel(GROUPS, HERE, regex_func_num, input_stream_tensor, basic_stream_tensor, cfg)\n")
+        self.gen_kernel_module_file(["kernel_0.py"])
+        return wrapper

@@ -1,0 +1,240 @@
+// libbitgen_hip.so -- C ABI of the MI355X "hip" backend (see include/bitgen_hip.h).
+//
+// Kernels owned by the runtime (the group kernels are generated per pattern
+// set and loaded as code objects):
+//   bg_transpose_kernel  bytes -> 8 basis streams (LSB-first u64 words)
+//   bg_msb32_kernel      u64 LSB-first -> the reference's u32 MSB-first layout
+//   bg_popcount_kernel   match counting
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "bitgen_hip.h"
+#include "transpose_core.h"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int set_err(int code, const char *what) {
+  snprintf(g_err, sizeof(g_err), "%s", what);
+  return code;
+}
+
+int hip_err(hipError_t e, const char *what) {
+  snprintf(g_err, sizeof(g_err), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+  return (int)e;
+}
+
+#define BG_HIP(call, what)                \
+  do {                                    \
+    hipError_t _e = (call);               \
+    if (_e != hipSuccess) return hip_err(_e, what); \
+  } while (0)
+
+inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+struct bg_module_s {
+  hipModule_t mod;
+};
+
+struct bg_kernel_s {
+  hipFunction_t fn;
+};
+
+// ---------------------------------------------------------------------------
+// Transpose: thread = one 64-byte block (one word of each basis stream) of one
+// input.  Full blocks are read as four 16-byte loads; the tail block is read
+// byte by byte with bounds checks; blocks past the data write zero words.
+__global__ __launch_bounds__(256) void bg_transpose_kernel(const uint8_t *__restrict__ bytes,
+                                                           uint64_t n_bytes, uint64_t in_stride,
+                                                           uint64_t *__restrict__ basis,
+                                                           uint64_t n_words) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t input = blockIdx.y;
+  if (w >= n_words) return;
+  const uint8_t *src = bytes + (uint64_t)input * in_stride;
+  uint64_t *dst = basis + (uint64_t)input * 8 * n_words;
+  const uint64_t off = w * 64;
+  uint64_t x[8], y[8];
+  if (off + 64 <= n_bytes) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(src + off);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint4 v = p[k];
+      x[2 * k] = ((uint64_t)v.y << 32) | v.x;
+      x[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      uint64_t v = 0;
+      for (int b = 0; b < 8; b++) {
+        const uint64_t pos = off + 8 * j + b;
+        if (pos < n_bytes) v |= (uint64_t)src[pos] << (8 * b);
+      }
+      x[j] = v;
+    }
+  }
+  bg_transpose_block(x, y);
+#pragma unroll
+  for (int i = 0; i < 8; i++) dst[(uint64_t)i * n_words + w] = y[i];
+}
+
+// dst word 2w   = positions 64w..64w+31 with position p at bit 31 - p%32,
+// dst word 2w+1 = positions 64w+32..64w+63.
+__global__ __launch_bounds__(256) void bg_msb32_kernel(const uint64_t *__restrict__ src,
+                                                       uint32_t *__restrict__ dst, uint64_t total) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t v = src[i];
+  dst[2 * i] = __builtin_bitreverse32((uint32_t)v);
+  dst[2 * i + 1] = __builtin_bitreverse32((uint32_t)(v >> 32));
+}
+
+__global__ __launch_bounds__(256) void bg_popcount_kernel(const uint64_t *__restrict__ src, uint64_t n,
+                                                          unsigned long long *count) {
+  uint64_t acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    acc += __popcll(src[i]);
+  // wave reduction through DPP-free shuffles
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(count, (unsigned long long)acc);
+}
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int bg_init(int device) {
+  BG_HIP(hipSetDevice(device), "hipSetDevice");
+  BG_HIP(hipFree(nullptr), "runtime init");
+  g_err[0] = 0;
+  return 0;
+}
+
+const char *bg_last_error(void) { return g_err; }
+
+int bg_transpose(const uint8_t *bytes, uint64_t n_bytes, uint64_t in_stride, uint64_t *basis,
+                 uint64_t n_words, uint32_t n_inputs, void *stream) {
+  if (!basis || (n_bytes && !bytes)) return set_err(BG_EINVAL, "bg_transpose: null pointer");
+  if (n_words * 64 < n_bytes) return set_err(BG_EINVAL, "bg_transpose: n_words * 64 < n_bytes");
+  if (n_inputs == 0 || n_words == 0) return 0;
+  if (n_inputs > 1 && in_stride < n_bytes) return set_err(BG_EINVAL, "bg_transpose: in_stride < n_bytes");
+  if ((reinterpret_cast<uintptr_t>(bytes) & 15) || (n_inputs > 1 && (in_stride & 15)))
+    return set_err(BG_EINVAL, "bg_transpose: input must be 16-byte aligned");
+  if (n_inputs > 65535) return set_err(BG_EINVAL, "bg_transpose: too many inputs");
+  const uint64_t blocks = (n_words + 255) / 256;
+  if (blocks > 0x7fffffffull) return set_err(BG_EINVAL, "bg_transpose: input too large");
+  hipLaunchKernelGGL(bg_transpose_kernel, dim3((uint32_t)blocks, n_inputs), dim3(256), 0, S(stream),
+                     bytes, n_bytes, in_stride, basis, n_words);
+  BG_HIP(hipGetLastError(), "bg_transpose launch");
+  return 0;
+}
+
+int bg_module_load(const char *path, bg_module_t *out) {
+  if (!path || !out) return set_err(BG_EINVAL, "bg_module_load: null argument");
+  hipModule_t m;
+  BG_HIP(hipModuleLoad(&m, path), "hipModuleLoad");
+  *out = new bg_module_s{m};
+  return 0;
+}
+
+int bg_module_load_data(const void *image, bg_module_t *out) {
+  if (!image || !out) return set_err(BG_EINVAL, "bg_module_load_data: null argument");
+  hipModule_t m;
+  BG_HIP(hipModuleLoadData(&m, image), "hipModuleLoadData");
+  *out = new bg_module_s{m};
+  return 0;
+}
+
+int bg_module_unload(bg_module_t module) {
+  if (!module) return 0;
+  hipError_t e = hipModuleUnload(module->mod);
+  delete module;
+  if (e != hipSuccess) return hip_err(e, "hipModuleUnload");
+  return 0;
+}
+
+int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out) {
+  if (!module || !name || !out) return set_err(BG_EINVAL, "bg_kernel_get: null argument");
+  hipFunction_t f;
+  hipError_t e = hipModuleGetFunction(&f, module->mod, name);
+  if (e != hipSuccess) {
+    snprintf(g_err, sizeof(g_err), "kernel %s not found: %s", name, hipGetErrorString(e));
+    return BG_ENOTFOUND;
+  }
+  *out = new bg_kernel_s{f};
+  return 0;
+}
+
+int bg_kernel_info(bg_kernel_t k, bg_kernel_info_t *info) {
+  if (!k || !info) return set_err(BG_EINVAL, "bg_kernel_info: null argument");
+  int v = -1;
+  info->vgprs = -1;
+  info->sgprs = -1;
+  if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, k->fn) == hipSuccess) info->vgprs = v;
+  v = 0;
+  BG_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, k->fn), "shared size");
+  info->lds_bytes = v;
+  BG_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, k->fn), "local size");
+  info->scratch_bytes = v;
+  BG_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn), "max threads");
+  info->max_threads = v;
+  return 0;
+}
+
+int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t *waves_launched,
+                    void *stream) {
+  if (!k || !args) return set_err(BG_EINVAL, "bg_group_launch: null argument");
+  if (!args->basis || !args->result || !args->status || !args->counter)
+    return set_err(BG_EINVAL, "bg_group_launch: null buffer");
+  if (args->n_tiles == 0) {
+    if (waves_launched) *waves_launched = 0;
+    return 0;
+  }
+  if (args->epoch == 0 || args->epoch >= (1u << 28))
+    return set_err(BG_EINVAL, "bg_group_launch: epoch must be in 1 .. 2^28-1");
+  const uint32_t waves_per_block = 4;
+  const uint32_t blocks = (args->n_tiles + waves_per_block - 1) / waves_per_block;
+  bg_group_args_t a = *args;
+  void *params[] = {&a};
+  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, 64 * waves_per_block, 1, 1, 0, S(stream), params,
+                               nullptr),
+         "hipModuleLaunchKernel");
+  if (waves_launched) *waves_launched = blocks * waves_per_block;
+  return 0;
+}
+
+int bg_to_reference_layout(const uint64_t *src, uint32_t *dst, uint64_t n_words, uint64_t n_streams,
+                           void *stream) {
+  if (!src || !dst) return set_err(BG_EINVAL, "bg_to_reference_layout: null pointer");
+  const uint64_t total = n_words * n_streams;
+  if (total == 0) return 0;
+  const uint64_t blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffffull) return set_err(BG_EINVAL, "bg_to_reference_layout: too large");
+  hipLaunchKernelGGL(bg_msb32_kernel, dim3((uint32_t)blocks), dim3(256), 0, S(stream), src, dst, total);
+  BG_HIP(hipGetLastError(), "bg_to_reference_layout launch");
+  return 0;
+}
+
+int bg_popcount(const uint64_t *src, uint64_t n_words, uint64_t *count, void *stream) {
+  if (!src || !count) return set_err(BG_EINVAL, "bg_popcount: null pointer");
+  if (n_words == 0) return 0;
+  uint64_t blocks = (n_words + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(bg_popcount_kernel, dim3((uint32_t)blocks), dim3(256), 0, S(stream), src, n_words,
+                     reinterpret_cast<unsigned long long *>(count));
+  BG_HIP(hipGetLastError(), "bg_popcount launch");
+  return 0;
+}
+
+int bg_stream_sync(void *stream) {
+  BG_HIP(hipStreamSynchronize(S(stream)), "hipStreamSynchronize");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+/*
+ * bitgen_hip.h -- C ABI of libbitgen_hip.so, the MI355X (gfx950) runtime of
+ * the BitGen "hip" backend.  Plain pointers and sizes only; device buffers
+ * are owned by the caller (PyTorch tensors on the Python side).  Every call
+ * returns 0 on success or a nonzero status (a hipError_t value, or one of the
+ * BG_E* codes below); bg_last_error() gives the message of the calling
+ * thread's last failure.  Streams are passed as `void *` (a hipStream_t;
+ * NULL = the default stream).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef BITGEN_HIP_H
+#define BITGEN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BG_EINVAL 1001   /* bad argument (shape, alignment, null pointer) */
+#define BG_ENOTFOUND 1002 /* kernel name not in module */
+
+typedef struct bg_module_s *bg_module_t;
+typedef struct bg_kernel_s *bg_kernel_t;
+
+/* Launch descriptor of a generated group kernel (one group over one buffer).
+ * Field order is part of the ABI: generated kernels take it by value. */
+typedef struct bg_group_args {
+  const uint64_t *basis; /* [8][n_words] basis streams, LSB-first words      */
+  uint64_t *result;      /* [n_words] match-end stream of the group          */
+  uint32_t *status;      /* [n_tiles][n_carry] carry look-back status words  */
+  uint32_t *counter;     /* [0] tile counter (monotonic, never reset),
+                            [1] error flag, set nonzero if a carry look-back
+                            wait gives up (results are then invalid)         */
+  uint64_t n_words;      /* words per stream                                 */
+  uint32_t n_tiles;      /* tiles covering n_words                           */
+  uint32_t tile_base;    /* value of *counter when this launch starts        */
+  uint32_t epoch;        /* launch epoch, 1 .. 2^28-1, distinct per launch   */
+  uint32_t n_carry;      /* carry operations in the program (>= 1 slot)      */
+} bg_group_args_t;
+
+typedef struct bg_kernel_info {
+  int32_t vgprs;        /* architected VGPRs per lane (-1 if unknown)   */
+  int32_t sgprs;
+  int32_t lds_bytes;    /* static LDS                                    */
+  int32_t scratch_bytes;/* private (spill) bytes per lane                */
+  int32_t max_threads;  /* max threads per block                         */
+} bg_kernel_info_t;
+
+/* Select the device and warm up the runtime.  Replaces the implicit CUDA
+ * context set-up of CUDA_Kernel (bitgen/backend/cuda/cuda_kernel.py:50-63). */
+int bg_init(int device);
+
+/* Message of this thread's last failed call ("" if none). */
+const char *bg_last_error(void);
+
+/* Byte -> basis transpose.  Replaces bitgen.cuda.byte_stream_transpose
+ * (bitgen/template/cuda/pybind.cpp:9, bitgen/template/cuda/kernel.cu:102-124).
+ * bytes: [n_inputs][in_stride] device bytes, the first n_bytes of each used;
+ * basis: [n_inputs][8][n_words] words, position p = bit p%64 of word p/64,
+ * basis i = bit (7 - i) of each byte; positions >= n_bytes are zero.
+ * Requires n_words * 64 >= n_bytes.  Asynchronous on `stream` (the reference
+ * synchronises the device on every call, kernel.cu:123). */
+int bg_transpose(const uint8_t *bytes, uint64_t n_bytes, uint64_t in_stride,
+                 uint64_t *basis, uint64_t n_words, uint32_t n_inputs,
+                 void *stream);
+
+/* Load a gfx950 code object (.hsaco).  Replaces the NVRTC/nvJitLink build and
+ * cuModuleLoadData of CUDA_Kernel.init (cuda_kernel.py:64-288). */
+int bg_module_load(const char *hsaco_path, bg_module_t *out);
+int bg_module_load_data(const void *image, bg_module_t *out);
+int bg_module_unload(bg_module_t module);
+
+/* Look up a kernel by name (cuModuleGetFunction, cuda_kernel.py:153-171). */
+int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out);
+
+/* Resource usage (cuda_kernel.py:296-398 prints the CUDA equivalents). */
+int bg_kernel_info(bg_kernel_t kernel, bg_kernel_info_t *info);
+
+/* Launch one generated group kernel over one buffer: grid of ceil(waves/4)
+ * blocks of 256 threads.  Replaces CUDA_Kernel.run (cuda_kernel.py:411-427),
+ * asynchronously: the caller synchronises.  The caller must advance
+ * args->tile_base by the returned wave count (*waves_launched) before the
+ * next launch sharing the counter. */
+int bg_group_launch(bg_kernel_t kernel, const bg_group_args_t *args,
+                    uint32_t *waves_launched, void *stream);
+
+/* LSB-first u64 words -> the reference's u32 layout (position p = bit
+ * 31 - p%32 of word p/32, bitgen/bitstream.py:30-52).  dst holds 2*n_words
+ * u32 words per stream; n_streams streams back to back. */
+int bg_to_reference_layout(const uint64_t *src, uint32_t *dst, uint64_t n_words,
+                           uint64_t n_streams, void *stream);
+
+/* Popcount of n_words words, accumulated into *count (device u64). */
+int bg_popcount(const uint64_t *src, uint64_t n_words, uint64_t *count, void *stream);
+
+int bg_stream_sync(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+"""Bit-exact parity of the hip path against the oracle (GPU)."""
+import random
+
+import numpy as np
+import pytest
+
+from bitgen_amd import workloads
+from oracle import oracle as O
+from tests.gpu_helpers import assert_parity
+
+pytestmark = pytest.mark.gpu
+
+# regexes whose unbounded repeats are over single classes (MatchStar form);
+# repeats of longer sub-expressions need while loops, which the hip backend
+# rejects (see test_codegen_rejects_general_loops)
+POOL = ["ab", "a[bc]+d", "x.*y", "^q", "z$", "(ab|cd){1,3}e", "b[^\\n]*c", "a(bc){0,3}d", "c?d",
+        "[0-9]{2,4}", "abcdx", "q(a|b)z*[xy]{2}", "a.{0,3}b", "(a|bb){1,4}c", "dd[^a]*c",
+        "^a[0-9]*$", "x[yz]+q*0"]
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 127, 128, 4095, 4096, 4097, 100003, 1 << 20])
+def test_transpose_matches_oracle(gpu, n):
+    import torch
+    from bitgen_amd.bitstream import transpose_bytes
+
+    rng = np.random.default_rng(n)
+    data = rng.integers(0, 256, size=n, dtype=np.uint8)
+    x = torch.from_numpy(data).to(gpu).reshape(1, 1, -1) if n else torch.zeros((1, 1, 0), dtype=torch.uint8, device=gpu)
+    basis, _ = transpose_bytes(x, n)
+    torch.cuda.synchronize()
+    got = basis.cpu().numpy()[0, 0]
+    ref = O.transpose(data, got.shape[-1])
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_groups(gpu, seed):
+    rng = random.Random(seed)
+    alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
+    n = rng.choice([50, 1000, 4000, 9000, 70000, 300001])
+    data = np.random.default_rng(seed).choice(alpha, size=n)
+    groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(1, 3))]
+    print(f"seed={seed} n={n} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, repeat=2, label=f"seed {seed}")
+
+
+def test_plain_small_workload(gpu):
+    regexes, n_groups, buf = workloads.plain_small()
+    from bitgen_amd.compiler import group_regex_in_list
+
+    groups = group_regex_in_list(regexes, n_groups)
+    ref = assert_parity(groups, buf, gpu, label="plain_small")
+    assert O.popcount(ref) > 0
+
+
+def test_carry_crosses_many_tiles(gpu):
+    # one 300 KB line: MatchStar/ScanThru carries and the line-end projection
+    # propagate through ~75 tiles
+    body = np.full(300_000, ord("x"), dtype=np.uint8)
+    body[0] = ord("a")
+    body[-2] = ord("b")
+    body[-1] = ord("\n")
+    body[150_000] = ord("a")
+    groups = [["a.*b", "a[^\\n]*b"], ["ax*b"], ["q"]]
+    ref = assert_parity(groups, body, gpu, label="long line")
+    assert O.popcount(ref[0]) == 1 and O.popcount(ref[1]) == 1 and O.popcount(ref[2]) == 0
+
+
+def test_long_shifts_use_multi_word_halo(gpu):
+    rng = np.random.default_rng(3)
+    data = rng.choice(np.frombuffer(b"abcx\n", dtype=np.uint8), size=200_000, p=[.2, .2, .2, .39, .01])
+    groups = [["a.{0,100}b"], ["a.{150}b", "c[ab]{70,90}c"], ["ab.{0,40}ba.{0,40}ab.{0,40}c"]]
+    assert_parity(groups, data, gpu, label="long shifts")
