@@ -30,6 +30,39 @@ class GroupArgs(ctypes.Structure):
     ]
 
 
+class Job(ctypes.Structure):
+    """Mirror of ``bg_job_t``."""
+
+    _fields_ = [
+        ("basis", ctypes.c_void_p),
+        ("result", ctypes.c_void_p),
+        ("status", ctypes.c_void_p),
+        ("n_words", ctypes.c_uint64),
+        ("group", ctypes.c_uint32),
+        ("n_carry", ctypes.c_uint32),
+        ("tile_start", ctypes.c_uint32),
+        ("n_tiles", ctypes.c_uint32),
+    ]
+
+
+class BatchArgs(ctypes.Structure):
+    """Mirror of ``bg_batch_args_t``."""
+
+    _fields_ = [
+        ("jobs", ctypes.c_void_p),
+        ("tile_job", ctypes.c_void_p),
+        ("ctl", ctypes.c_void_p),
+        ("n_jobs", ctypes.c_uint32),
+        ("total_tiles", ctypes.c_uint32),
+    ]
+
+
+class TDesc(ctypes.Structure):
+    """Mirror of ``bg_tdesc_t``."""
+
+    _fields_ = [(f, ctypes.c_uint64) for f in ("byte_off", "n_bytes", "basis_off", "n_words", "word_start")]
+
+
 class KernelInfo(ctypes.Structure):
     _fields_ = [
         ("vgprs", ctypes.c_int32),
@@ -43,7 +76,8 @@ class KernelInfo(ctypes.Structure):
 EXPORTS = (
     "bg_init", "bg_last_error", "bg_transpose", "bg_module_load", "bg_module_load_data",
     "bg_module_unload", "bg_kernel_get", "bg_kernel_info", "bg_group_launch",
-    "bg_to_reference_layout", "bg_popcount", "bg_stream_sync",
+    "bg_to_reference_layout", "bg_popcount", "bg_stream_sync", "bg_batch_launch",
+    "bg_graph_begin", "bg_graph_end", "bg_graph_launch", "bg_graph_destroy", "bg_transpose_batch",
 )
 
 _lib = None
@@ -79,6 +113,12 @@ def load_library():
         L.bg_to_reference_layout.argtypes = [vp, vp, u64, u64, vp]
         L.bg_popcount.argtypes = [vp, u64, vp, vp]
         L.bg_stream_sync.argtypes = [vp]
+        L.bg_batch_launch.argtypes = [vp, ctypes.POINTER(BatchArgs), u32, u32, vp]
+        L.bg_graph_begin.argtypes = [vp]
+        L.bg_graph_end.argtypes = [vp, ctypes.POINTER(vp)]
+        L.bg_graph_launch.argtypes = [vp, vp]
+        L.bg_graph_destroy.argtypes = [vp]
+        L.bg_transpose_batch.argtypes = [vp, vp, vp, u32, u64, vp]
         for name in EXPORTS:
             if name != "bg_last_error":
                 getattr(L, name).restype = ctypes.c_int
@@ -138,10 +178,11 @@ class Module:
 
 
 def group_launch(kernel, args, stream):
-    waves = ctypes.c_uint32()
-    _check(load_library().bg_group_launch(kernel, ctypes.byref(args), ctypes.byref(waves), stream),
+    """Returns how far the launch advances the tile counter."""
+    adv = ctypes.c_uint32()
+    _check(load_library().bg_group_launch(kernel, ctypes.byref(args), ctypes.byref(adv), stream),
            "bg_group_launch")
-    return waves.value
+    return adv.value
 
 
 def to_reference_layout(src_ptr, dst_ptr, n_words, n_streams, stream):
@@ -155,3 +196,38 @@ def popcount(src_ptr, n_words, count_ptr, stream):
 
 def stream_sync(stream):
     _check(load_library().bg_stream_sync(stream), "bg_stream_sync")
+
+
+def batch_launch(kernel, args, grid_blocks, block_threads, stream):
+    _check(load_library().bg_batch_launch(kernel, ctypes.byref(args), grid_blocks, block_threads, stream),
+           "bg_batch_launch")
+
+
+class Graph:
+    """A captured, instantiated graph of work enqueued on one stream."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream):
+        _check(load_library().bg_graph_launch(self.handle, stream), "bg_graph_launch")
+
+    def destroy(self):
+        if self.handle:
+            load_library().bg_graph_destroy(self.handle)
+            self.handle = None
+
+
+def graph_begin(stream):
+    _check(load_library().bg_graph_begin(stream), "bg_graph_begin")
+
+
+def graph_end(stream):
+    h = ctypes.c_void_p()
+    _check(load_library().bg_graph_end(stream, ctypes.byref(h)), "bg_graph_end")
+    return Graph(h)
+
+
+def transpose_batch(bytes_ptr, basis_ptr, descs_ptr, n_desc, total_words, stream):
+    _check(load_library().bg_transpose_batch(bytes_ptr, basis_ptr, descs_ptr, n_desc, total_words, stream),
+           "bg_transpose_batch")

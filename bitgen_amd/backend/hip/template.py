@@ -88,9 +88,9 @@ class HipGroupSet:
         args.tile_base = self.tile_base[g] & 0xFFFFFFFF
         args.epoch = epoch
         args.n_carry = max(1, meta["n_carry"])
-        waves = rt.group_launch(self.kernels[g], args, stream)
-        self.tile_base[g] = (self.tile_base[g] + waves) & 0xFFFFFFFF
-        return waves
+        adv = rt.group_launch(self.kernels[g], args, stream)
+        self.tile_base[g] = (self.tile_base[g] + adv) & 0xFFFFFFFF
+        return adv
 
     def run(self, basis, result, status, stream=None):
         """basis u64 [S=1, M, 8, W]; result u64 [1, M, G, W]; status int32 [M, G, *]."""

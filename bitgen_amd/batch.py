@@ -1,0 +1,252 @@
+"""Batched execution: many regex groups x many buffers of uneven length.
+
+The reference launches one grid of (groups x inputs) blocks per pattern set
+over copies of one input (``gen_cuda.py:441-473``, ``run_bitgen.py:239-245``);
+uneven buffers need one launch each.  Here all buffers of a batch are
+transposed by one launch (a descriptor per buffer) and every (group, buffer)
+job of a pattern set runs in one persistent launch of the set's batch kernel,
+whose waves take 4,096-position tiles from a device work queue
+(``tile_job``): one persistent launch per group of each set covers all
+buffers.  The sequence (one transpose launch + one batch launch per group) can
+be captured into a graph and replayed.
+
+Layout (all device-resident, allocated once per plan):
+  bytes   u8  [sum of 64-byte-aligned buffer lengths]
+  basis   u64 [sum over buffers of 8 * W_b]
+  result  u64 [sum over jobs of W_b]       (per job: the group's match stream)
+  status  u32 [sum over jobs of T_j * C_g]
+  jobs    bg_job_t[n_jobs], tile_job u32[total tiles]    (per set and group)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .backend.hip import runtime as rt
+from .codegen.gen_hip import GroupProgram, batch_kernel_name, batch_source, compile_sources
+from .compiler import DEFAULT_OPTIONS, group_programs, _set_dir
+
+WAVE_WORDS = 64
+
+
+def words_for(n):
+    return (n + 1 + 63) // 64
+
+
+def n_tiles(meta, W):
+    s0 = 64 - meta["HR"]
+    if W <= s0:
+        return 1
+    return 1 + (W - s0 + meta["S"] - 1) // meta["S"]
+
+
+class BatchPatternSet:
+    """The groups of one pattern set, each compiled into a persistent batch
+    kernel (separate code objects, compiled in parallel)."""
+
+    def __init__(self, groups, options=None, block_threads=256):
+        o = dict(DEFAULT_OPTIONS)
+        o.update(options or {})
+        self.groups = [list(g) for g in groups]
+        self.options = o
+        self.block_threads = block_threads
+        programs = []
+        for gid, regs in enumerate(self.groups):
+            _, _, post, vnm = group_programs(regs, o)
+            programs.append(GroupProgram(gid, post, vnm, regs))
+        self.metas = [p.meta() for p in programs]
+        d = _set_dir(self.groups, dict(o, form="batch", block=block_threads))
+        sources = {}
+        for p in programs:
+            path = os.path.join(d, f"batch_{p.gid}.hip")
+            src = batch_source(p, block_threads)
+            with open(path, "w") as f:
+                f.write(src)
+            sources[path] = src
+        hs = compile_sources(sources)
+        self.hsacos = [os.path.abspath(hs[os.path.join(d, f"batch_{p.gid}.hip")]) for p in programs]
+        self.modules = None
+        self.kernels = None
+
+    @property
+    def n_groups(self):
+        return len(self.groups)
+
+    def load(self, device=0):
+        rt.init(device)
+        if self.modules is None:
+            self.modules = [rt.Module(h) for h in self.hsacos]
+            self.kernels = [m.kernel(batch_kernel_name(g)) for g, m in enumerate(self.modules)]
+        return self
+
+    def info(self):
+        return [m.info(batch_kernel_name(g)) for g, m in enumerate(self.modules)]
+
+    def free(self):
+        if self.modules is not None:
+            for m in self.modules:
+                m.free()
+            self.modules = None
+            self.kernels = None
+
+
+class BatchPlan:
+    """Device buffers and work queues for a fixed batch of jobs.
+
+    ``sets``: list of loaded BatchPatternSet; ``buffers``: list of uint8
+    arrays; ``jobs``: list of (set index, buffer index) pairs.  Each pair
+    expands to one job per group of the set."""
+
+    def __init__(self, sets, buffers, jobs, device=0, grid_blocks=None):
+        import torch
+
+        self.torch = torch
+        self.dev = torch.device("cuda", device)
+        self.sets = sets
+        self.n_buffers = len(buffers)
+        self.lengths = [int(np.asarray(b).size) for b in buffers]
+        self.words = [words_for(n) for n in self.lengths]
+        # bytes: 64-byte aligned offsets
+        offs, o = [], 0
+        for n in self.lengths:
+            offs.append(o)
+            o += (n + 63) // 64 * 64
+        self.byte_off = offs
+        host = np.zeros(max(o, 64), dtype=np.uint8)
+        for b, off in zip(buffers, offs):
+            a = np.asarray(b, dtype=np.uint8)
+            host[off:off + a.size] = a
+        self.bytes = torch.from_numpy(host).to(self.dev)
+        boffs, o = [], 0
+        for W in self.words:
+            boffs.append(o)
+            o += 8 * W
+        self.basis_off = boffs
+        self.basis = torch.zeros(max(o, 1), dtype=torch.uint64, device=self.dev)
+        # one work queue per (set, group): the group's jobs over the buffers
+        self.jobs = list(jobs)
+        res_total = 0
+        stat_total = 0
+        self.queues = []   # (set, group) -> queue state
+        for si, ps in enumerate(sets):
+            bufs = [b for (s, b) in self.jobs if s == si]
+            for g, meta in enumerate(ps.metas):
+                C = max(1, meta["n_carry"])
+                entries, tile = [], 0
+                for b in bufs:
+                    W = self.words[b]
+                    T = n_tiles(meta, W)
+                    entries.append(dict(set=si, group=g, buffer=b, W=W, T=T, C=C, tile_start=tile,
+                                        res_off=res_total, stat_off=stat_total))
+                    res_total += W
+                    stat_total += T * C
+                    tile += T
+                self.queues.append(dict(set=si, group=g, entries=entries, total_tiles=tile))
+        self.result = torch.zeros(max(res_total, 1), dtype=torch.uint64, device=self.dev)
+        self.status = torch.zeros(max(stat_total, 1), dtype=torch.int32, device=self.dev)
+        for q in self.queues:
+            entries = q["entries"]
+            arr = (rt.Job * max(1, len(entries)))()
+            for k, e in enumerate(entries):
+                arr[k].basis = self.basis.data_ptr() + 8 * self.basis_off[e["buffer"]]
+                arr[k].result = self.result.data_ptr() + 8 * e["res_off"]
+                arr[k].status = self.status.data_ptr() + 4 * e["stat_off"]
+                arr[k].n_words = e["W"]
+                arr[k].group = e["group"]
+                arr[k].n_carry = e["C"]
+                arr[k].tile_start = e["tile_start"]
+                arr[k].n_tiles = e["T"]
+            q["jobs"] = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(self.dev)
+            tj = (np.repeat(np.arange(len(entries), dtype=np.int64), [e["T"] for e in entries])
+                  if entries else np.zeros(1, dtype=np.int64))
+            q["tile_job"] = torch.from_numpy(tj.astype(np.int32)).to(self.dev)
+            q["ctl"] = torch.zeros(4, dtype=torch.int32, device=self.dev)
+            args = rt.BatchArgs()
+            args.jobs = q["jobs"].data_ptr()
+            args.tile_job = q["tile_job"].data_ptr()
+            args.ctl = q["ctl"].data_ptr()
+            args.n_jobs = len(entries)
+            args.total_tiles = q["total_tiles"]
+            q["args"] = args
+        # one transpose launch for all buffers (a descriptor per buffer)
+        td = (rt.TDesc * max(1, self.n_buffers))()
+        ws = 0
+        for b in range(self.n_buffers):
+            td[b].byte_off = self.byte_off[b]
+            td[b].n_bytes = self.lengths[b]
+            td[b].basis_off = self.basis_off[b]
+            td[b].n_words = self.words[b]
+            td[b].word_start = ws
+            ws += self.words[b]
+        self.total_words = ws
+        self.tdesc = torch.from_numpy(np.frombuffer(bytes(td), dtype=np.uint8).copy()).to(self.dev)
+        self.grid_blocks = grid_blocks
+        self.graph = None
+
+    # -- execution -------------------------------------------------------------
+    def _grid(self, ps, total_tiles):
+        waves_per_block = ps.block_threads // 64
+        if self.grid_blocks:
+            blocks = self.grid_blocks
+        else:
+            props = self.torch.cuda.get_device_properties(self.dev)
+            blocks = max(1, props.multi_processor_count * 16 // waves_per_block)
+        return max(1, min(blocks, (total_tiles + waves_per_block - 1) // waves_per_block))
+
+    def enqueue_transpose(self, stream):
+        rt.transpose_batch(self.bytes.data_ptr(), self.basis.data_ptr(), self.tdesc.data_ptr(), self.n_buffers,
+                           self.total_words, stream)
+
+    def enqueue_match(self, stream):
+        for q in self.queues:
+            ps = self.sets[q["set"]]
+            if q["total_tiles"]:
+                rt.batch_launch(ps.kernels[q["group"]], q["args"], self._grid(ps, q["total_tiles"]),
+                                ps.block_threads, stream)
+
+    def enqueue(self, stream=None):
+        s = rt.current_stream_ptr(stream)
+        self.enqueue_transpose(s)
+        self.enqueue_match(s)
+
+    def capture(self, stream=None):
+        """Capture transpose + batch launches into a graph (replay with run_graph)."""
+        torch = self.torch
+        cap = torch.cuda.Stream(device=self.dev)
+        cap.wait_stream(torch.cuda.current_stream(self.dev))
+        s = rt.current_stream_ptr(cap)
+        rt.graph_begin(s)
+        try:
+            self.enqueue_transpose(s)
+            self.enqueue_match(s)
+        finally:
+            self.graph = rt.graph_end(s)
+        torch.cuda.current_stream(self.dev).wait_stream(cap)
+        return self.graph
+
+    def run_graph(self, stream=None):
+        if self.graph is None:
+            self.capture()
+        self.graph.launch(rt.current_stream_ptr(stream))
+
+    def check_errors(self):
+        for q in self.queues:
+            if int(q["ctl"][1].item()) != 0:
+                raise rt.HipRuntimeError("carry look-back timed out in a batched launch; results are invalid")
+
+    def results(self, set_index, buffer_index):
+        """Match streams [G, W] of one (set, buffer) job (device tensor view)."""
+        rows = [e for q in self.queues if q["set"] == set_index for e in q["entries"] if e["buffer"] == buffer_index]
+        if not rows:
+            raise KeyError((set_index, buffer_index))
+        W = rows[0]["W"]
+        rows.sort(key=lambda e: e["group"])
+        return self.torch.stack([self.result[e["res_off"]:e["res_off"] + W] for e in rows])
+
+    def scanned_bytes(self):
+        return sum(self.lengths[b] for (_, b) in self.jobs)
+
+    def destroy(self):
+        if self.graph is not None:
+            self.graph.destroy()
+            self.graph = None

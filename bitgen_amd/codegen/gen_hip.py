@@ -136,26 +136,26 @@ class GroupProgram:
             return "~0ull"
         return _cname(v)
 
-    def source(self):
+    def device_function(self):
+        """The group program for one tile as a device function (shared by the
+        plain per-group kernel and the batched kernel)."""
         lines = []
         add = lines.append
         regex_note = json.dumps(self.regexes)[:2000].replace("*/", "* /")
-        add('#include "bitgen_device.h"')
         add(f"/* group {self.gid}: {regex_note} */")
-        add(f'extern "C" __global__ __launch_bounds__(256) void {self.kernel_name}(bg_group_args_t a) {{')
+        add(f"static __device__ __forceinline__ void bg_prog_{self.gid}(const uint64_t *__restrict__ basis, "
+            "uint64_t *__restrict__ result, uint32_t *status, int64_t W, uint32_t t, uint32_t epoch, "
+            "uint32_t n_carry, uint32_t *err) {")
         add(f"  constexpr int HL = {self.HL}, HR = {self.HR}, S = {self.S};")
-        add("  const uint32_t t = bg_acquire_tile(a.counter, a.tile_base);")
-        add("  if (t >= a.n_tiles) return;")
         add("  const int lane = bg_lane();")
-        add("  const int64_t W = (int64_t)a.n_words;")
         add("  const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;")
         add("  const int64_t w = w0 + lane;")
         add("  const bool inb = (w >= 0) && (w < W);")
-        add("  const bg_tile tc{t, w0, W, a.status, a.n_carry, a.epoch, a.counter + 1};")
+        add("  const bg_tile tc{t, w0, W, status, n_carry, epoch, err};")
         add("  (void)tc;")
         add("  uint64_t bas[8];")
         add("#pragma unroll")
-        add("  for (int i = 0; i < 8; i++) bas[i] = inb ? a.basis[(int64_t)i * W + w] : 0ull;")
+        add("  for (int i = 0; i < 8; i++) bas[i] = inb ? basis[(int64_t)i * W + w] : 0ull;")
         # declarations (goto must not cross initialisations)
         vars_ = set()
         for inst in self.insts:
@@ -170,6 +170,13 @@ class GroupProgram:
             add(f"  uint64_t {chunk};")
         add("  uint64_t out = 0;")
         carry_index = {id(inst): (k, L) for k, (inst, L) in enumerate(self.carry_ops)}
+        # values shared by every regex (the basis and the group header defined
+        # before the first zero test) are fenced at regex boundaries
+        first_skip = next((k for k, i in enumerate(self.insts) if i.type == BsInstType.SKIPZERO), len(self.insts))
+        header = [d for i in self.insts[:first_skip] for d in i.defs()]
+        later = {u for i in self.insts[first_skip:] for u in i.uses()}
+        fence_vars = [v for v in dict.fromkeys(header) if v in later]
+        open_regions = set()
         for inst in self.insts:
             t = inst.type
             if t == BsInstType.STREAMLOAD:
@@ -194,8 +201,14 @@ class GroupProgram:
                     f"{self.val(inst.operands[1])}, tc);")
             elif t == BsInstType.SKIPZERO:
                 add(f"  if (bg_wave_zero({self.val(inst.operands[0])})) goto L{inst.label};")
+                open_regions.add(inst.label)
             elif t == BsInstType.LABEL:
                 add(f"L{inst.label}:;")
+                open_regions.discard(inst.label)
+                if not open_regions:
+                    # end of a regex's skip regions: later classes are
+                    # recomputed instead of kept live from here
+                    add("  " + " ".join(f"bg_fence({_cname(v)});" for v in fence_vars))
             elif t == BsInstType.STREAMSTORE:
                 add(f"  out = {self.val(inst.operands[0])};")
             elif t == BsInstType.GRAPHBREAK:
@@ -204,9 +217,26 @@ class GroupProgram:
                 raise HipCodegenError(f"unsupported instruction {inst}")
         add("  const int lo = (t == 0) ? 0 : HL;")
         add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
-        add("  if (inb && lane >= lo && lane < hi) a.result[w] = out;")
+        add("  if (inb && lane >= lo && lane < hi) result[w] = out;")
         add("}")
         return "\n".join(lines) + "\n"
+
+    def source(self):
+        """Plain form: one kernel per group, one launch per (group, buffer)."""
+        return "\n".join([
+            '#include "bitgen_device.h"',
+            self.device_function(),
+            f'extern "C" __global__ __launch_bounds__(256) void {self.kernel_name}(bg_group_args_t a) {{',
+            "  // persistent: each wave takes tiles until none is left, so the",
+            "  // program's executed code stays in the instruction cache",
+            "  while (true) {",
+            "    const uint32_t t = bg_acquire_tile(a.counter, a.tile_base);",
+            "    if (t >= a.n_tiles) return;",
+            f"    bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, a.epoch, a.n_carry, a.counter + 1);",
+            "  }",
+            "}",
+            "",
+        ])
 
     def meta(self):
         return {
@@ -219,6 +249,33 @@ class GroupProgram:
             "n_insts": len(self.insts),
             "regexes": list(self.regexes),
         }
+
+
+def batch_kernel_name(gid):
+    return f"bg_batch_{gid}"
+
+
+def batch_source(program, block_threads=256):
+    """Batched form of one group: a persistent kernel whose waves take tiles
+    from a device work queue over many buffers (global tile index -> job
+    through ``tile_job``; each job's tiles are contiguous and increasing, so a
+    tile's predecessor always has a smaller index)."""
+    g = program.gid
+    return "\n".join([
+        '#include "bitgen_device.h"',
+        program.device_function(),
+        f'extern "C" __global__ __launch_bounds__({block_threads}) void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
+        "  const uint32_t epoch = a.ctl[2];",
+        "  while (true) {",
+        "    const uint32_t tg = bg_acquire_tile(a.ctl, 0);",
+        "    if (tg >= a.total_tiles) return;",
+        "    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
+        "    const bg_job_t J = a.jobs[j];",
+        f"    bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, epoch, J.n_carry, a.ctl + 1);",
+        "  }",
+        "}",
+        "",
+    ])
 
 
 def _hsaco_cache_dir():

@@ -16,6 +16,7 @@
 namespace {
 
 thread_local char g_err[512] = "";
+int g_cu_count = 256;  // compute units of the current device (set by bg_init)
 
 int set_err(int code, const char *what) {
   snprintf(g_err, sizeof(g_err), "%s", what);
@@ -44,6 +45,21 @@ struct bg_module_s {
 struct bg_kernel_s {
   hipFunction_t fn;
 };
+
+struct bg_graph_s {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+// Resets the tile counter and advances the epoch (1 .. 2^28-2) of a batched
+// launch on the device, so a captured graph replays with fresh status tags.
+__global__ void bg_batch_prologue_kernel(uint32_t *ctl) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    ctl[0] = 0;
+    const uint32_t e = ctl[2];
+    ctl[2] = (e >= (1u << 28) - 2) ? 1u : e + 1u;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Transpose: thread = one 64-byte block (one word of each basis stream) of one
@@ -84,6 +100,49 @@ __global__ __launch_bounds__(256) void bg_transpose_kernel(const uint8_t *__rest
   for (int i = 0; i < 8; i++) dst[(uint64_t)i * n_words + w] = y[i];
 }
 
+// Batched transpose: global word index -> buffer by binary search over the
+// descriptors' word_start, then the same per-block transpose.
+__global__ __launch_bounds__(256) void bg_transpose_batch_kernel(const uint8_t *__restrict__ bytes,
+                                                                 uint64_t *__restrict__ basis,
+                                                                 const bg_tdesc_t *__restrict__ descs,
+                                                                 uint32_t n_desc, uint64_t total_words) {
+  const uint64_t gw = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gw >= total_words) return;
+  uint32_t lo = 0, hi = n_desc - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (descs[mid].word_start <= gw) lo = mid; else hi = mid - 1;
+  }
+  const bg_tdesc_t d = descs[lo];
+  const uint64_t w = gw - d.word_start;
+  const uint8_t *src = bytes + d.byte_off;
+  uint64_t *dst = basis + d.basis_off;
+  const uint64_t off = w * 64;
+  uint64_t x[8], y[8];
+  if (off + 64 <= d.n_bytes) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(src + off);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint4 v = p[k];
+      x[2 * k] = ((uint64_t)v.y << 32) | v.x;
+      x[2 * k + 1] = ((uint64_t)v.w << 32) | v.z;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      uint64_t v = 0;
+      for (int b = 0; b < 8; b++) {
+        const uint64_t pos = off + 8 * j + b;
+        if (pos < d.n_bytes) v |= (uint64_t)src[pos] << (8 * b);
+      }
+      x[j] = v;
+    }
+  }
+  bg_transpose_block(x, y);
+#pragma unroll
+  for (int i = 0; i < 8; i++) dst[(uint64_t)i * d.n_words + w] = y[i];
+}
+
 // dst word 2w   = positions 64w..64w+31 with position p at bit 31 - p%32,
 // dst word 2w+1 = positions 64w+32..64w+63.
 __global__ __launch_bounds__(256) void bg_msb32_kernel(const uint64_t *__restrict__ src,
@@ -112,6 +171,9 @@ extern "C" {
 int bg_init(int device) {
   BG_HIP(hipSetDevice(device), "hipSetDevice");
   BG_HIP(hipFree(nullptr), "runtime init");
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    g_cu_count = cus;
   g_err[0] = 0;
   return 0;
 }
@@ -132,6 +194,20 @@ int bg_transpose(const uint8_t *bytes, uint64_t n_bytes, uint64_t in_stride, uin
   hipLaunchKernelGGL(bg_transpose_kernel, dim3((uint32_t)blocks, n_inputs), dim3(256), 0, S(stream),
                      bytes, n_bytes, in_stride, basis, n_words);
   BG_HIP(hipGetLastError(), "bg_transpose launch");
+  return 0;
+}
+
+int bg_transpose_batch(const uint8_t *bytes, uint64_t *basis, const bg_tdesc_t *descs, uint32_t n_desc,
+                       uint64_t total_words, void *stream) {
+  if (total_words == 0 || n_desc == 0) return 0;
+  if (!bytes || !basis || !descs) return set_err(BG_EINVAL, "bg_transpose_batch: null pointer");
+  if (reinterpret_cast<uintptr_t>(bytes) & 15)
+    return set_err(BG_EINVAL, "bg_transpose_batch: byte array must be 16-byte aligned");
+  const uint64_t blocks = (total_words + 255) / 256;
+  if (blocks > 0x7fffffffull) return set_err(BG_EINVAL, "bg_transpose_batch: too large");
+  hipLaunchKernelGGL(bg_transpose_batch_kernel, dim3((uint32_t)blocks), dim3(256), 0, S(stream), bytes, basis,
+                     descs, n_desc, total_words);
+  BG_HIP(hipGetLastError(), "bg_transpose_batch launch");
   return 0;
 }
 
@@ -187,25 +263,29 @@ int bg_kernel_info(bg_kernel_t k, bg_kernel_info_t *info) {
   return 0;
 }
 
-int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t *waves_launched,
+int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t *counter_advance,
                     void *stream) {
   if (!k || !args) return set_err(BG_EINVAL, "bg_group_launch: null argument");
   if (!args->basis || !args->result || !args->status || !args->counter)
     return set_err(BG_EINVAL, "bg_group_launch: null buffer");
   if (args->n_tiles == 0) {
-    if (waves_launched) *waves_launched = 0;
+    if (counter_advance) *counter_advance = 0;
     return 0;
   }
   if (args->epoch == 0 || args->epoch >= (1u << 28))
     return set_err(BG_EINVAL, "bg_group_launch: epoch must be in 1 .. 2^28-1");
+  // persistent waves: enough to fill every CU, never more than the tiles
   const uint32_t waves_per_block = 4;
-  const uint32_t blocks = (args->n_tiles + waves_per_block - 1) / waves_per_block;
+  const uint32_t max_blocks = (uint32_t)g_cu_count * 8;
+  uint32_t blocks = (args->n_tiles + waves_per_block - 1) / waves_per_block;
+  if (blocks > max_blocks) blocks = max_blocks;
   bg_group_args_t a = *args;
   void *params[] = {&a};
   BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, 64 * waves_per_block, 1, 1, 0, S(stream), params,
                                nullptr),
          "hipModuleLaunchKernel");
-  if (waves_launched) *waves_launched = blocks * waves_per_block;
+  // every tile is taken once and every wave makes one final, failing take
+  if (counter_advance) *counter_advance = args->n_tiles + blocks * waves_per_block;
   return 0;
 }
 
@@ -229,6 +309,56 @@ int bg_popcount(const uint64_t *src, uint64_t n_words, uint64_t *count, void *st
   hipLaunchKernelGGL(bg_popcount_kernel, dim3((uint32_t)blocks), dim3(256), 0, S(stream), src, n_words,
                      reinterpret_cast<unsigned long long *>(count));
   BG_HIP(hipGetLastError(), "bg_popcount launch");
+  return 0;
+}
+
+int bg_batch_launch(bg_kernel_t k, const bg_batch_args_t *args, uint32_t grid_blocks, uint32_t block_threads,
+                    void *stream) {
+  if (!k || !args || !args->jobs || !args->tile_job || !args->ctl)
+    return set_err(BG_EINVAL, "bg_batch_launch: null argument");
+  if (block_threads == 0 || block_threads % 64 || block_threads > 1024)
+    return set_err(BG_EINVAL, "bg_batch_launch: block_threads must be a multiple of 64, at most 1024");
+  if (grid_blocks == 0) return set_err(BG_EINVAL, "bg_batch_launch: grid_blocks must be positive");
+  hipLaunchKernelGGL(bg_batch_prologue_kernel, dim3(1), dim3(64), 0, S(stream), args->ctl);
+  BG_HIP(hipGetLastError(), "batch prologue launch");
+  if (args->total_tiles == 0) return 0;
+  bg_batch_args_t a = *args;
+  void *params[] = {&a};
+  BG_HIP(hipModuleLaunchKernel(k->fn, grid_blocks, 1, 1, block_threads, 1, 1, 0, S(stream), params, nullptr),
+         "hipModuleLaunchKernel(batch)");
+  return 0;
+}
+
+int bg_graph_begin(void *stream) {
+  BG_HIP(hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  return 0;
+}
+
+int bg_graph_end(void *stream, bg_graph_t *out) {
+  if (!out) return set_err(BG_EINVAL, "bg_graph_end: null argument");
+  hipGraph_t g;
+  BG_HIP(hipStreamEndCapture(S(stream), &g), "hipStreamEndCapture");
+  hipGraphExec_t e;
+  hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  if (err != hipSuccess) {
+    hipGraphDestroy(g);
+    return hip_err(err, "hipGraphInstantiate");
+  }
+  *out = new bg_graph_s{g, e};
+  return 0;
+}
+
+int bg_graph_launch(bg_graph_t graph, void *stream) {
+  if (!graph) return set_err(BG_EINVAL, "bg_graph_launch: null graph");
+  BG_HIP(hipGraphLaunch(graph->exec, S(stream)), "hipGraphLaunch");
+  return 0;
+}
+
+int bg_graph_destroy(bg_graph_t graph) {
+  if (!graph) return 0;
+  hipGraphExecDestroy(graph->exec);
+  hipGraphDestroy(graph->graph);
+  delete graph;
   return 0;
 }
 

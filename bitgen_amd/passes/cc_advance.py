@@ -35,7 +35,7 @@ knob is accepted and recorded only.
 from ..inst import (
     BsInstType, BsAssign, BsAdvance, BsNot, ADV_LEFT, ADV_RIGHT, walk,
 )
-from .pass_utils import add_new_var, is_const
+from .pass_utils import is_const
 
 
 class cc_advance_pass:
@@ -43,6 +43,8 @@ class cc_advance_pass:
         self.pass_name = "cc_advance"
         self.advance_count = {}
         self.shift_cache = {}   # (var, amount, left) -> shifted var id
+        self.cache_keys = {}    # var -> cache keys naming it (as source or result)
+        self.names = None
         self.inserted_left = 0
         self.inserted_right = 0
 
@@ -51,21 +53,37 @@ class cc_advance_pass:
 
     # -- helpers -------------------------------------------------------------
     def _invalidate(self, var):
-        for key in [k for k, v in self.shift_cache.items() if k[0] == var or v == var]:
-            del self.shift_cache[key]
+        for key in self.cache_keys.pop(var, ()):
+            self.shift_cache.pop(key, None)
+
+    def _new_var(self, vnm, name):
+        if self.names is None:
+            self.names = set(vnm.values())
+            self.next_id = max(vnm) + 1 if vnm else 0
+        cand, n = name, 0
+        while cand in self.names:
+            n += 1
+            cand = f"{name}_{n}"
+        vid = self.next_id
+        self.next_id += 1
+        vnm[vid] = cand
+        self.names.add(cand)
+        return vid
 
     def _shift(self, out, vnm, var, amount, left):
         key = (var, amount, left)
         if key in self.shift_cache:
             return self.shift_cache[key]
         name = f"{vnm[var]}_advance{amount}" if left else f"{vnm[var]}_realign{amount}"
-        new = add_new_var(vnm, name)
+        new = self._new_var(vnm, name)
         out.append(BsAdvance([var, amount], new, ADV_LEFT if left else ADV_RIGHT))
         if left:
             self.inserted_left += 1
         else:
             self.inserted_right += 1
         self.shift_cache[key] = new
+        self.cache_keys.setdefault(var, []).append(key)
+        self.cache_keys.setdefault(new, []).append(key)
         return new
 
     def _define(self, ret, count):

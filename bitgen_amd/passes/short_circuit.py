@@ -26,7 +26,7 @@ finds chains with ``nx.all_simple_paths`` from the basis loads (exponential,
 
 Only straight-line top-level code is scanned (loops and branches end a range).
 """
-from ..inst import BsInstType, BsAssign, BsSkipZero, BsLabel
+from ..inst import BsInstType, BsAssign, BsOr, BsSkipZero, BsLabel
 from .pass_utils import def_counts, top_level_last_use, const_id, is_const
 from .. import config as cfg
 
@@ -96,33 +96,89 @@ class short_circuit_pass:
             return insts, vnm
 
         zero = const_id(vnm, "0")
-        before = {}   # index -> list of instructions inserted before it
-        after = {}    # index -> labels placed after it
-        for label, (i, e) in enumerate(regions):
-            v = insts[i].ret
+        live_of = []
+        for (i, e) in regions:
             live = []
             for j in range(i + 1, e + 1):
                 for x in insts[j].defs():
                     if last_use.get(x, -1) > e and x not in live:
                         live.append(x)
-            pre = [BsAssign([zero], x) for x in live]
+            live_of.append(live)
+        folded = self._fold_accumulations(insts, regions, live_of, defs, last_use)
+
+        before = {}   # index -> list of instructions inserted before it
+        after = {}    # index -> instructions / labels placed after it
+        zeroed = {}   # value -> check index of the range that zero-initialises it
+        for label, (i, e) in enumerate(regions):
+            v = insts[i].ret
+            live = live_of[label]
+            # x is written once, inside this range; a range opened earlier
+            # that already zeroes x has not reached x's definition yet, so x is
+            # still zero here
+            pre = [BsAssign([zero], x) for x in live if not (x in zeroed and zeroed[x] < i)]
+            for x in live:
+                zeroed.setdefault(x, i)
             pre.append(BsSkipZero(v, label))
             before.setdefault(i + 1, []).extend(pre)
             after.setdefault(e, []).append(BsLabel(label))
             self.checks += 1
             self.skippable_insts += e - i
+        for e, (first_check, init, acc) in folded.items():
+            before.setdefault(first_check + 1, []).insert(0, init)
+            after[e].insert(0, acc)
         out = []
         for j, inst in enumerate(insts):
+            if j - 1 in folded:
+                continue  # accumulation moved into the range ending at j - 1
             out.extend(before.get(j, []))
             out.append(inst)
-            # inner (later-opened) ranges close first
-            out.extend(reversed(after.get(j, [])))
+            # the moved accumulation first, then labels (inner ranges close first)
+            tail = after.get(j, [])
+            if j in folded:
+                out.append(tail[0])
+                tail = tail[1:]
+            out.extend(reversed(tail))
         return out, vnm
 
+    @staticmethod
+    def _fold_accumulations(insts, regions, live_of, defs, last_use):
+        """``y = a | z`` right after ranges whose only result z feeds it becomes
+        ``y = a`` before the first check and ``y = y | z`` at the end of the
+        ranges: skipping then leaves y = a = a | 0 in place, so the
+        accumulator needs no zero-initialised z and no copy at the join."""
+        ends = {}
+        for k, (i, e) in enumerate(regions):
+            ends.setdefault(e, []).append(k)
+        folded = {}
+        for e, ks in ends.items():
+            if e + 1 >= len(insts):
+                continue
+            acc = insts[e + 1]
+            if acc.type != BsInstType.OR or defs.get(acc.ret, 0) != 1:
+                continue
+            first = min(regions[k][0] for k in ks)
+            # no other range may span the accumulation
+            if any(i < e + 1 <= ee and ee != e for (i, ee) in regions):
+                continue
+            a, b = acc.operands
+            for z, other in ((a, b), (b, a)):
+                if (all(z in live_of[k] for k in ks) and last_use.get(z) == e + 1
+                        and other not in {d for j in range(first + 1, e + 1) for d in insts[j].defs()}
+                        and other != acc.ret and z != other):
+                    for k in ks:
+                        live_of[k].remove(z)
+                    y = acc.ret
+                    folded[e] = (first, BsAssign([other], y), BsOr([y, z], y))
+                    break
+        return folded
+
     def _region(self, insts, i, defs, last_use, vnm):
+        import heapq
+
         v = insts[i].ret
         Z = {v}
-        defined = []
+        pending = set()   # defined in the range, not in Z, read after the current end
+        heap = []         # (last use, value) of pending values
         best = None
         useful = 0
         horizon = last_use.get(v, i)
@@ -148,8 +204,15 @@ class short_circuit_pass:
                 Z.update(ds)
                 for x in ds:
                     horizon = max(horizon, last_use.get(x, j))
-            defined.extend(ds)
+            else:
+                for x in ds:
+                    lu = last_use.get(x, -1)
+                    if lu > j:
+                        pending.add(x)
+                        heapq.heappush(heap, (lu, x))
+            while heap and heap[0][0] <= j:
+                pending.discard(heapq.heappop(heap)[1])
             useful += 1
-            if all(x in Z or last_use.get(x, -1) <= j for x in defined):
-                best = j if useful >= self.min_region else best
+            if not pending and useful >= self.min_region:
+                best = j
         return best

@@ -25,74 +25,72 @@
 
 #define BG_WAVE 64
 
-// ---- cross-lane moves -------------------------------------------------------
-// Value of lane (l - 1); lane 0 receives 0.
-__device__ __forceinline__ uint64_t bg_lane_prev1(uint64_t x) {
-  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-  lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x138 /*wave_shr:1*/, 0xF, 0xF, false);
-  hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x138, 0xF, 0xF, false);
-  return ((uint64_t)hi << 32) | lo;
-}
-
-// Value of lane (l + 1); lane 63 receives 0.
-__device__ __forceinline__ uint64_t bg_lane_next1(uint64_t x) {
-  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
-  lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x130 /*wave_shl:1*/, 0xF, 0xF, false);
-  hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x130, 0xF, 0xF, false);
-  return ((uint64_t)hi << 32) | lo;
-}
-
+// ---- cross-lane moves (32-bit halves) ----------------------------------------
 __device__ __forceinline__ int bg_lane() { return (int)__lane_id(); }
 
-// Value of lane (l - K) for K >= 0, zero where l - K < 0.
+// Value of lane (l - K); zero where l - K < 0.
 template <int K>
-__device__ __forceinline__ uint64_t bg_lane_prev(uint64_t x) {
+__device__ __forceinline__ uint32_t bg_prev32(uint32_t x) {
   if constexpr (K == 0) {
     return x;
   } else if constexpr (K == 1) {
-    return bg_lane_prev1(x);
+    // bound_ctrl: lane 0 (no source lane) reads 0, no initialised destination needed
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x138 /*wave_shr:1*/, 0xF, 0xF, true);
   } else if constexpr (K >= BG_WAVE) {
-    return 0;
+    return 0u;
   } else {
     const int l = bg_lane();
-    const int src = (l - K) & (BG_WAVE - 1);
-    uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)x);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(x >> 32));
-    uint64_t v = ((uint64_t)hi << 32) | lo;
-    return (l >= K) ? v : 0ull;
+    const uint32_t v = (uint32_t)__builtin_amdgcn_ds_bpermute(((l - K) & (BG_WAVE - 1)) << 2, (int)x);
+    return (l >= K) ? v : 0u;
   }
 }
 
-// Value of lane (l + K) for K >= 0, zero where l + K > 63.
+// Value of lane (l + K); zero where l + K > 63.
 template <int K>
-__device__ __forceinline__ uint64_t bg_lane_next(uint64_t x) {
+__device__ __forceinline__ uint32_t bg_next32(uint32_t x) {
   if constexpr (K == 0) {
     return x;
   } else if constexpr (K == 1) {
-    return bg_lane_next1(x);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x130 /*wave_shl:1*/, 0xF, 0xF, true);
   } else if constexpr (K >= BG_WAVE) {
-    return 0;
+    return 0u;
   } else {
     const int l = bg_lane();
-    const int src = (l + K) & (BG_WAVE - 1);
-    uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)x);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)(uint32_t)(x >> 32));
-    uint64_t v = ((uint64_t)hi << 32) | lo;
-    return (l + K < BG_WAVE) ? v : 0ull;
+    const uint32_t v = (uint32_t)__builtin_amdgcn_ds_bpermute(((l + K) & (BG_WAVE - 1)) << 2, (int)x);
+    return (l + K < BG_WAVE) ? v : 0u;
   }
+}
+
+__device__ __forceinline__ uint32_t bg_lo(uint64_t x) { return (uint32_t)x; }
+__device__ __forceinline__ uint32_t bg_hi(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ uint64_t bg_cat(uint32_t hi, uint32_t lo) { return ((uint64_t)hi << 32) | lo; }
+
+// ((a:b) >> s)[31:0] for s in 0..31 (v_alignbit_b32)
+__device__ __forceinline__ uint32_t bg_align(uint32_t a, uint32_t b, uint32_t s) {
+  return __builtin_amdgcn_alignbit(a, b, s);
 }
 
 // ---- shifts ------------------------------------------------------------------
-// Advance by N (toward higher positions): y[p] = x[p - N].
+// Advance by N (toward higher positions): y[p] = x[p - N].  Words come from
+// lanes l - Q and l - Q - 1 (Q = N / 64); only the 32-bit halves that reach
+// the result are moved, and the funnel is two v_alignbit_b32.
 template <int N>
 __device__ __forceinline__ uint64_t bg_advr(uint64_t x) {
   constexpr int Q = N / 64, R = N % 64;
+  const uint32_t xl = bg_lo(x), xh = bg_hi(x);
   if constexpr (R == 0) {
-    return bg_lane_prev<Q>(x);
+    return bg_cat(bg_prev32<Q>(xh), bg_prev32<Q>(xl));
+  } else if constexpr (R < 32) {
+    const uint32_t cl = bg_prev32<Q>(xl), ch = bg_prev32<Q>(xh);
+    const uint32_t ph = bg_prev32<Q + 1>(xh);
+    return bg_cat(bg_align(ch, cl, 32 - R), bg_align(cl, ph, 32 - R));
+  } else if constexpr (R == 32) {
+    return bg_cat(bg_prev32<Q>(xl), bg_prev32<Q + 1>(xh));
   } else {
-    const uint64_t cur = bg_lane_prev<Q>(x);
-    const uint64_t prev = bg_lane_prev<Q + 1>(x);
-    return (cur << R) | (prev >> (64 - R));
+    constexpr int r = R - 32;
+    const uint32_t cl = bg_prev32<Q>(xl);
+    const uint32_t pl = bg_prev32<Q + 1>(xl), ph = bg_prev32<Q + 1>(xh);
+    return bg_cat(bg_align(cl, ph, 32 - r), bg_align(ph, pl, 32 - r));
   }
 }
 
@@ -100,14 +98,27 @@ __device__ __forceinline__ uint64_t bg_advr(uint64_t x) {
 template <int N>
 __device__ __forceinline__ uint64_t bg_advl(uint64_t x) {
   constexpr int Q = N / 64, R = N % 64;
+  const uint32_t xl = bg_lo(x), xh = bg_hi(x);
   if constexpr (R == 0) {
-    return bg_lane_next<Q>(x);
+    return bg_cat(bg_next32<Q>(xh), bg_next32<Q>(xl));
+  } else if constexpr (R < 32) {
+    const uint32_t cl = bg_next32<Q>(xl), ch = bg_next32<Q>(xh);
+    const uint32_t nl = bg_next32<Q + 1>(xl);
+    return bg_cat(bg_align(nl, ch, R), bg_align(ch, cl, R));
+  } else if constexpr (R == 32) {
+    return bg_cat(bg_next32<Q + 1>(xl), bg_next32<Q>(xh));
   } else {
-    const uint64_t cur = bg_lane_next<Q>(x);
-    const uint64_t nxt = bg_lane_next<Q + 1>(x);
-    return (cur >> R) | (nxt << (64 - R));
+    constexpr int r = R - 32;
+    const uint32_t ch = bg_next32<Q>(xh);
+    const uint32_t nl = bg_next32<Q + 1>(xl), nh = bg_next32<Q + 1>(xh);
+    return bg_cat(bg_align(nh, nl, r), bg_align(nl, ch, r));
   }
 }
+
+// Keeps the compiler from sharing (and keeping live) values computed from x
+// before this point with values computed after it: issued at regex
+// boundaries so every regex recomputes its classes next to their use.
+__device__ __forceinline__ void bg_fence(uint64_t &x) { asm volatile("" : "+v"(x)); }
 
 // ---- zero tests ------------------------------------------------------------
 __device__ __forceinline__ bool bg_wave_zero(uint64_t x) {

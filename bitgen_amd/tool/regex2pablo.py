@@ -8,7 +8,9 @@ project's own producer of programs in the same dialect and conventions:
 
 * basis streams ``basis0..7`` with basis0 = most significant bit of each byte
   (``bitgen/template/template.py:135-145``);
-* character classes named ``CC__<hex ranges>``, loop variables ``test*`` /
+* character classes named ``CC__<hex ranges>`` (by default built from 32
+  shared nibble masks in the group header, ``cc_mode="nibble"``; or as a
+  decision diagram over the basis bits, ``cc_mode="bdd"``), loop variables ``test*`` /
   ``accum*`` (the names ``bitgen/passes/remove_alias.py:24`` and
   ``bitgen/passes/cc_advance.py:106`` key on), result in ``output.matches``;
 * markers sit one past the consumed character: consuming class C from marker
@@ -301,11 +303,16 @@ def cc_name(mask):
 class PabloEmitter:
     """Builds one pablo program for a group of regexes."""
 
-    def __init__(self, star_mode="matchstar", line_mode=True):
+    def __init__(self, star_mode="matchstar", line_mode=True, cc_mode="nibble"):
         if star_mode not in ("matchstar", "while"):
             raise ValueError(star_mode)
+        if cc_mode not in ("nibble", "bdd"):
+            raise ValueError(cc_mode)
         self.star_mode = star_mode
         self.line_mode = line_mode
+        self.cc_mode = cc_mode
+        self.nib_hi = None
+        self.nib_lo = None
         self.lines = []
         self.indent = ""
         self.counter = {}
@@ -423,13 +430,70 @@ class PabloEmitter:
         got = self.memo_get(key)
         if got:
             return got
-        # Values are indexed with basis0 (the MSB) as the top split: value v's
-        # bit within the level-0 mask is v itself.
-        fn = self._cc_fn(0, mask)
+        if self.cc_mode == "nibble" and self.nib_hi is not None:
+            fn = self._cc_nibble(mask)
+        else:
+            # Values are indexed with basis0 (the MSB) as the top split: value
+            # v's bit within the level-0 mask is v itself.
+            fn = self._cc_fn(0, mask)
         if fn in ("0", "1"):
             return fn
         name = self.emit(self.fresh(cc_name(mask)), fn)
         return self.memo_put(key, name)
+
+    # -- nibble-factored classes -------------------------------------------------
+    def emit_nibble_masks(self):
+        """Group header: H[h] = (high nibble == h), L[l] = (low nibble == l),
+        built from 2-bit products of the basis streams (56 operations), so a
+        single-byte class is one AND: CC(b) = H[b >> 4] & L[b & 15]."""
+        nb = [self.NOT(f"basis{i}") for i in range(8)]
+
+        def pair(i, k):
+            a = f"basis{i}" if (k >> 1) & 1 else nb[i]
+            b = f"basis{i + 1}" if k & 1 else nb[i + 1]
+            return self.AND(a, b)
+
+        p01 = [pair(0, k) for k in range(4)]
+        p23 = [pair(2, k) for k in range(4)]
+        p45 = [pair(4, k) for k in range(4)]
+        p67 = [pair(6, k) for k in range(4)]
+        self.nib_hi = [self.emit(self.fresh(f"nib_hi_{h:x}"), f"({p01[h >> 2]} & {p23[h & 3]})") for h in range(16)]
+        self.nib_lo = [self.emit(self.fresh(f"nib_lo_{l:x}"), f"({p45[l >> 2]} & {p67[l & 3]})") for l in range(16)]
+
+    def _low_fn(self, lowmask):
+        """Function of basis4..7 selecting the low nibbles in the 16-bit mask."""
+        n = bin(lowmask).count("1")
+        if n == 16:
+            return "1"
+        if n <= 2:
+            out = "0"
+            for l in range(16):
+                if lowmask >> l & 1:
+                    out = self.OR(out, self.nib_lo[l])
+            return out
+        if n >= 14:
+            out = "0"
+            for l in range(16):
+                if not lowmask >> l & 1:
+                    out = self.OR(out, self.nib_lo[l])
+            return self.NOT(out)
+        return self._cc_fn(4, lowmask)
+
+    def _cc_nibble(self, mask):
+        if bin(mask).count("1") > 128:
+            return self.NOT(self._cc_nibble(ALL & ~mask))
+        rows = {}
+        for h in range(16):
+            row = (mask >> (16 * h)) & 0xFFFF
+            if row:
+                rows.setdefault(row, []).append(h)
+        out = "0"
+        for row, hs in sorted(rows.items(), key=lambda kv: kv[1][0]):
+            hi = "0"
+            for h in hs:
+                hi = self.OR(hi, self.nib_hi[h])
+            out = self.OR(out, self.AND(hi, self._low_fn(row)))
+        return out
 
     # -- group-level streams --------------------------------------------------
     def lf(self):
@@ -550,8 +614,14 @@ class PabloEmitter:
     def emit_group(self, regexes):
         asts = [parse_regex(r) for r in regexes]
         group_scope = self.group_memo.setdefault("_scope", {})
+        # group header: shared streams are defined before any regex code
+        self.scopes = [group_scope]
+        if self.cc_mode == "nibble":
+            self.emit_nibble_masks()
         if self.line_mode:
             self.lf()
+        if any(_has_start(a) for a in asts):
+            self.sol()
         acc = "0"
         for ast in asts:
             # common subexpressions are shared inside one regex only, so every
@@ -576,8 +646,18 @@ class PabloEmitter:
         return "\n".join(self.lines) + "\n"
 
 
-def regexes_to_pablo(regexes, star_mode="matchstar", line_mode=True):
+def _has_start(node):
+    if isinstance(node, Start):
+        return True
+    if isinstance(node, (Seq, Alt)):
+        return any(_has_start(i) for i in node.items)
+    if isinstance(node, Rep):
+        return _has_start(node.node)
+    return False
+
+
+def regexes_to_pablo(regexes, star_mode="matchstar", line_mode=True, cc_mode="nibble"):
     """Pablo text for one group of regexes (strings without slashes)."""
     if isinstance(regexes, str):
         regexes = [regexes]
-    return PabloEmitter(star_mode=star_mode, line_mode=line_mode).emit_group(list(regexes))
+    return PabloEmitter(star_mode=star_mode, line_mode=line_mode, cc_mode=cc_mode).emit_group(list(regexes))

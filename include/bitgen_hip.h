@@ -40,6 +40,42 @@ typedef struct bg_group_args {
   uint32_t n_carry;      /* carry operations in the program (>= 1 slot)      */
 } bg_group_args_t;
 
+/* One (group, buffer) job of a batched launch.  Field order is ABI. */
+typedef struct bg_job {
+  const uint64_t *basis; /* [8][n_words] basis of the buffer               */
+  uint64_t *result;      /* [n_words] match stream of (group, buffer)       */
+  uint32_t *status;      /* [n_tiles][n_carry] look-back status words       */
+  uint64_t n_words;
+  uint32_t group;        /* group program index inside the batch kernel     */
+  uint32_t n_carry;      /* >= 1                                            */
+  uint32_t tile_start;   /* first global tile index of this job             */
+  uint32_t n_tiles;
+} bg_job_t;
+
+/* Launch descriptor of a generated batch kernel (all groups of a pattern set
+ * over many buffers).  Tiles are taken from a device queue: global tile index
+ * g belongs to job tile_job[g]; jobs are ordered so each job's tiles are
+ * contiguous and increasing. */
+typedef struct bg_batch_args {
+  const bg_job_t *jobs;
+  const uint32_t *tile_job; /* [total_tiles] job index of each global tile */
+  uint32_t *ctl;            /* [0] tile counter, [1] error flag, [2] epoch  */
+  uint32_t n_jobs;
+  uint32_t total_tiles;
+} bg_batch_args_t;
+
+typedef struct bg_graph_s *bg_graph_t;
+
+/* One buffer of a batched transpose.  word_start = sum of n_words of the
+ * buffers before it (the descriptors are sorted by it). */
+typedef struct bg_tdesc {
+  uint64_t byte_off;   /* offset of the buffer in the byte array (16-aligned) */
+  uint64_t n_bytes;
+  uint64_t basis_off;  /* word offset of the buffer's [8][n_words] basis      */
+  uint64_t n_words;
+  uint64_t word_start;
+} bg_tdesc_t;
+
 typedef struct bg_kernel_info {
   int32_t vgprs;        /* architected VGPRs per lane (-1 if unknown)   */
   int32_t sgprs;
@@ -66,6 +102,12 @@ int bg_transpose(const uint8_t *bytes, uint64_t n_bytes, uint64_t in_stride,
                  uint64_t *basis, uint64_t n_words, uint32_t n_inputs,
                  void *stream);
 
+/* Transpose of many buffers of uneven length in one launch (no counterpart
+ * in the reference, whose inputs are copies of one buffer).  descs: device
+ * array of n_desc descriptors; total_words = sum of their n_words. */
+int bg_transpose_batch(const uint8_t *bytes, uint64_t *basis, const bg_tdesc_t *descs, uint32_t n_desc,
+                       uint64_t total_words, void *stream);
+
 /* Load a gfx950 code object (.hsaco).  Replaces the NVRTC/nvJitLink build and
  * cuModuleLoadData of CUDA_Kernel.init (cuda_kernel.py:64-288). */
 int bg_module_load(const char *hsaco_path, bg_module_t *out);
@@ -78,13 +120,14 @@ int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out);
 /* Resource usage (cuda_kernel.py:296-398 prints the CUDA equivalents). */
 int bg_kernel_info(bg_kernel_t kernel, bg_kernel_info_t *info);
 
-/* Launch one generated group kernel over one buffer: grid of ceil(waves/4)
- * blocks of 256 threads.  Replaces CUDA_Kernel.run (cuda_kernel.py:411-427),
- * asynchronously: the caller synchronises.  The caller must advance
- * args->tile_base by the returned wave count (*waves_launched) before the
- * next launch sharing the counter. */
+/* Launch one generated group kernel over one buffer: persistent blocks of
+ * 256 threads (at most 8 per CU, at most one wave per tile) that take tiles
+ * from the counter until none is left.  Replaces CUDA_Kernel.run
+ * (cuda_kernel.py:411-427), asynchronously: the caller synchronises.  The
+ * caller must advance args->tile_base by *counter_advance before the next
+ * launch sharing the counter. */
 int bg_group_launch(bg_kernel_t kernel, const bg_group_args_t *args,
-                    uint32_t *waves_launched, void *stream);
+                    uint32_t *counter_advance, void *stream);
 
 /* LSB-first u64 words -> the reference's u32 layout (position p = bit
  * 31 - p%32 of word p/32, bitgen/bitstream.py:30-52).  dst holds 2*n_words
@@ -94,6 +137,22 @@ int bg_to_reference_layout(const uint64_t *src, uint32_t *dst, uint64_t n_words,
 
 /* Popcount of n_words words, accumulated into *count (device u64). */
 int bg_popcount(const uint64_t *src, uint64_t n_words, uint64_t *count, void *stream);
+
+/* Batched launch (no counterpart in the reference, which launches one grid
+ * of (groups x inputs) blocks per pattern set): enqueues a one-thread
+ * prologue kernel that resets ctl[0] and advances the epoch ctl[2] on the
+ * device -- so the pair can be captured in a graph and replayed -- then
+ * grid_blocks blocks of block_threads threads of the persistent batch kernel. */
+int bg_batch_launch(bg_kernel_t kernel, const bg_batch_args_t *args, uint32_t grid_blocks,
+                    uint32_t block_threads, void *stream);
+
+/* Stream capture into a replayable graph (hipStreamBeginCapture /
+ * hipStreamEndCapture / hipGraphInstantiate / hipGraphLaunch).  Work captured
+ * must not contain memsets (the batch prologue is a kernel for that reason). */
+int bg_graph_begin(void *stream);
+int bg_graph_end(void *stream, bg_graph_t *out);
+int bg_graph_launch(bg_graph_t graph, void *stream);
+int bg_graph_destroy(bg_graph_t graph);
 
 int bg_stream_sync(void *stream);
 
