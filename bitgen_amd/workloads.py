@@ -200,12 +200,60 @@ def genomic(n_bytes=1024 * 1024 * 1024, n_regex=512, n_groups=32, hit_rate=1e-6)
     return regexes, n_groups, buf
 
 
-def lognormal_sizes(n, seed, median, lo, hi, total=None):
-    g = np.random.default_rng(seed)
-    sigma = 1.5
-    sizes = np.exp(np.log(median) + sigma * g.standard_normal(n))
-    sizes = np.clip(sizes, lo, hi).astype(np.int64)
+def lognormal_sizes(n, seed, median, lo, hi, total=None, sigma=1.5):
+    """Log-normal lengths with the given median, clipped to [lo, hi].  With
+    ``total``, sigma is solved (bisection) so the clipped sum is ~total while
+    the median stays fixed."""
+    z = np.random.default_rng(seed).standard_normal(n)
+
+    def draw(sg):
+        return np.clip(np.exp(np.log(median) + sg * z), lo, hi).astype(np.int64)
+
     if total is not None:
-        scale = total / sizes.sum()
-        sizes = np.clip((sizes * scale).astype(np.int64), lo, hi)
-    return sizes
+        a, b = 0.01, 8.0
+        for _ in range(60):
+            m = (a + b) / 2
+            if draw(m).sum() < total:
+                a = m
+            else:
+                b = m
+        sigma = (a + b) / 2
+    return draw(sigma)
+
+
+def batched(n_buffers=4096, total_bytes=2 * 2**30, set_sizes=(64, 256, 1024), seed=10):
+    """Pattern sets of the ids mix (seeds 7, 8, 9) and buffers with log-normal
+    lengths (median 16 KiB, clipped to 1 KiB-8 MiB, scaled to ~total_bytes).
+    Jobs: every (set, buffer) pair."""
+    sets = [ids_regexes(n, 7 + i) for i, n in enumerate(set_sizes)]
+    sizes = lognormal_sizes(n_buffers, seed, 16 * 1024, 1024, 8 * 2**20, total=total_bytes)
+    g = np.random.default_rng(seed)
+    buffers = []
+    for i, n in enumerate(sizes):
+        buf = text_ids(int(n), seed * 100003 + i)
+        k = max(1, int(n * 1e-4))
+        pos = g.integers(0, max(1, int(n) - 40), size=k)
+        rs = sets[i % len(sets)]
+        for p in pos:
+            s_ = sample_match(rs[int(g.integers(0, len(rs)))], random.Random(int(p)))
+            buf[p:p + len(s_)] = np.frombuffer(s_, dtype=np.uint8)[:max(0, int(n) - int(p))]
+        buffers.append(buf)
+    jobs = [(si, b) for si in range(len(sets)) for b in range(n_buffers)]
+    return sets, buffers, jobs
+
+
+def groups_for(n_regex, per_group=64):
+    """Groups for a set size: about 64 regexes per group."""
+    return max(1, (n_regex + per_group - 1) // per_group)
+
+
+def serving(n_jobs, rate, seed=11, set_sizes=(32, 64, 128, 256, 512, 1024, 1536, 2048), median=4096,
+            max_bytes=64 * 2**20):
+    """Poisson arrivals at ``rate`` jobs/s: (arrival time, set index, buffer)."""
+    g = np.random.default_rng(seed)
+    gaps = g.exponential(1.0 / rate, size=n_jobs)
+    arrivals = np.cumsum(gaps)
+    sizes = lognormal_sizes(n_jobs, seed + 1, median, 64, max_bytes)
+    set_idx = g.integers(0, len(set_sizes), size=n_jobs)
+    bufs = [text_ids(int(n), seed * 7919 + i) for i, n in enumerate(sizes)]
+    return arrivals, set_idx, bufs
