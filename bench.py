@@ -35,10 +35,30 @@ def parse():
     ap.add_argument("--mib", type=int, default=100, help="buffer size in MiB")
     ap.add_argument("--regexes", type=int, default=1000)
     ap.add_argument("--groups", type=int, default=16)
+    ap.add_argument("--queues", type=int, default=0, help="tile queues per launch (0 = runtime default)")
+    ap.add_argument("--grid-blocks", type=int, default=0, help="persistent blocks per launch (0 = all that fit)")
     ap.add_argument("--cpu-sample-mib", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check the GPU result against the oracle on a slice")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (three times) and exit: the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size: reads and writes exactly
+    mib MiB per launch, so counter readings of it give the counter's scale."""
+    import torch
+
+    n = mib * 2**20
+    src = torch.randint(0, 255, (n,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n}), flush=True)
 
 
 def cpu_baseline(groups, buf, sample_bytes):
@@ -79,6 +99,9 @@ def main():
         print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is described in DESIGN.md "
                                    "but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     import numpy as np
     import torch
 
@@ -96,6 +119,13 @@ def main():
     print(f"[bench] compiled {len(groups)} group kernels ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     dev = torch.device("cuda", 0)
     ks = cs.load(0)
+    ks.n_queues = args.queues
+    ks.grid_blocks = args.grid_blocks
+    from bitgen_amd.backend.hip import runtime as rt_
+
+    occ = [rt_.kernel_occupancy(k, 256)[0] for k in ks.kernels]
+    print(f"[bench] blocks per CU (256 threads) per group kernel: {occ}; info {ks.info()[0]}", file=sys.stderr,
+          flush=True)
     x = torch.from_numpy(buf).to(dev).reshape(1, 1, -1)
     basis, staged = transpose_bytes(x, n_bytes)
     W = basis.size(-1)

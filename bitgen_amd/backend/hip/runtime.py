@@ -21,12 +21,12 @@ class GroupArgs(ctypes.Structure):
         ("basis", ctypes.c_void_p),
         ("result", ctypes.c_void_p),
         ("status", ctypes.c_void_p),
-        ("counter", ctypes.c_void_p),
+        ("ctl", ctypes.c_void_p),
         ("n_words", ctypes.c_uint64),
         ("n_tiles", ctypes.c_uint32),
-        ("tile_base", ctypes.c_uint32),
-        ("epoch", ctypes.c_uint32),
+        ("n_queues", ctypes.c_uint32),
         ("n_carry", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -54,7 +54,18 @@ class BatchArgs(ctypes.Structure):
         ("ctl", ctypes.c_void_p),
         ("n_jobs", ctypes.c_uint32),
         ("total_tiles", ctypes.c_uint32),
+        ("n_queues", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
+
+
+# Control block layout (BG_CTL_* in include/bitgen_hip.h): u32 word indices.
+MAX_QUEUES = 32
+CTL_STRIDE = 64
+CTL_QDONE = 2 * MAX_QUEUES * CTL_STRIDE
+CTL_ERROR = CTL_QDONE + CTL_STRIDE
+CTL_EPOCH = CTL_ERROR + CTL_STRIDE
+CTL_WORDS = CTL_EPOCH + CTL_STRIDE
 
 
 class TDesc(ctypes.Structure):
@@ -78,6 +89,7 @@ EXPORTS = (
     "bg_module_unload", "bg_kernel_get", "bg_kernel_info", "bg_group_launch",
     "bg_to_reference_layout", "bg_popcount", "bg_stream_sync", "bg_batch_launch",
     "bg_graph_begin", "bg_graph_end", "bg_graph_launch", "bg_graph_destroy", "bg_transpose_batch",
+    "bg_kernel_occupancy", "bg_ctl_words",
 )
 
 _lib = None
@@ -109,7 +121,9 @@ def load_library():
         L.bg_module_unload.argtypes = [vp]
         L.bg_kernel_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.bg_kernel_info.argtypes = [vp, ctypes.POINTER(KernelInfo)]
-        L.bg_group_launch.argtypes = [vp, ctypes.POINTER(GroupArgs), ctypes.POINTER(u32), vp]
+        L.bg_group_launch.argtypes = [vp, ctypes.POINTER(GroupArgs), u32, vp]
+        L.bg_kernel_occupancy.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.bg_ctl_words.argtypes = []
         L.bg_to_reference_layout.argtypes = [vp, vp, u64, u64, vp]
         L.bg_popcount.argtypes = [vp, u64, vp, vp]
         L.bg_stream_sync.argtypes = [vp]
@@ -120,8 +134,11 @@ def load_library():
         L.bg_graph_destroy.argtypes = [vp]
         L.bg_transpose_batch.argtypes = [vp, vp, vp, u32, u64, vp]
         for name in EXPORTS:
-            if name != "bg_last_error":
+            if name not in ("bg_last_error", "bg_ctl_words"):
                 getattr(L, name).restype = ctypes.c_int
+        L.bg_ctl_words.restype = u32
+        if L.bg_ctl_words() != CTL_WORDS:
+            raise HipRuntimeError(f"{path}: control block of {L.bg_ctl_words()} words, expected {CTL_WORDS}")
         _lib = L
         return L
 
@@ -177,12 +194,23 @@ class Module:
             self.handle = None
 
 
-def group_launch(kernel, args, stream):
-    """Returns how far the launch advances the tile counter."""
-    adv = ctypes.c_uint32()
-    _check(load_library().bg_group_launch(kernel, ctypes.byref(args), ctypes.byref(adv), stream),
-           "bg_group_launch")
-    return adv.value
+def group_launch(kernel, args, stream, grid_blocks=0):
+    _check(load_library().bg_group_launch(kernel, ctypes.byref(args), grid_blocks, stream), "bg_group_launch")
+
+
+def kernel_occupancy(kernel, block_threads):
+    """(blocks per CU, CU count) of a loaded kernel at a block size."""
+    n, cus = ctypes.c_int32(), ctypes.c_int32()
+    _check(load_library().bg_kernel_occupancy(kernel, block_threads, ctypes.byref(n), ctypes.byref(cus)),
+           "bg_kernel_occupancy")
+    return n.value, cus.value
+
+
+def new_ctl(device):
+    """A zeroed control block (int32 tensor of CTL_WORDS words) on a device."""
+    import torch
+
+    return torch.zeros(CTL_WORDS, dtype=torch.int32, device=device)
 
 
 def to_reference_layout(src_ptr, dst_ptr, n_words, n_streams, stream):

@@ -45,11 +45,11 @@ class HipGroupSet:
             self.modules.append(mod)
             self.kernels.append(mod.kernel(m["kernel"]))
         dev = torch.device("cuda", device)
-        # per group: [monotonic tile counter, look-back error flag] (launches of
-        # a group are ordered on one stream; groups never share a counter)
-        self.counters = torch.zeros((len(metas), 2), dtype=torch.int32, device=dev)
-        self.tile_base = [0] * len(metas)
-        self.epochs = {}
+        # one control block per group (tile queues, error flag, launch epoch):
+        # launches of a group are ordered on one stream, groups never share one
+        self.ctl = torch.zeros((len(metas), rt.CTL_WORDS), dtype=torch.int32, device=dev)
+        self.n_queues = 0  # 0 = the runtime's default
+        self.grid_blocks = 0  # 0 = as many blocks as fit on the device
 
     @property
     def n_groups(self):
@@ -66,31 +66,26 @@ class HipGroupSet:
         return max(self.n_tiles(m, n_words) * max(1, m["n_carry"]) for m in self.metas)
 
     def alloc_status(self, n_inputs, n_words):
+        """Zeroed look-back status words for n_inputs buffers of n_words words.
+        A status buffer belongs to this group set: its words are tagged with
+        the epochs of this set's control blocks."""
         torch = _torch()
         return torch.zeros((n_inputs, self.n_groups, self.status_words(n_words)), dtype=torch.int32,
                            device=torch.device("cuda", self.device))
 
-    def launch(self, g, basis_ptr, result_ptr, status_ptr, n_words, stream, status_key=None):
+    def launch(self, g, basis_ptr, result_ptr, status_ptr, n_words, stream):
         """Enqueue group g over one buffer (asynchronous)."""
         meta = self.metas[g]
-        key = (g, status_key if status_key is not None else status_ptr)
-        epoch = self.epochs.get(key, 0) + 1
-        if epoch >= (1 << 28):
-            epoch = 1
-        self.epochs[key] = epoch
         args = rt.GroupArgs()
         args.basis = basis_ptr
         args.result = result_ptr
         args.status = status_ptr
-        args.counter = self.counters.data_ptr() + 8 * g
+        args.ctl = self.ctl[g].data_ptr()
         args.n_words = n_words
         args.n_tiles = self.n_tiles(meta, n_words)
-        args.tile_base = self.tile_base[g] & 0xFFFFFFFF
-        args.epoch = epoch
+        args.n_queues = self.n_queues
         args.n_carry = max(1, meta["n_carry"])
-        adv = rt.group_launch(self.kernels[g], args, stream)
-        self.tile_base[g] = (self.tile_base[g] + adv) & 0xFFFFFFFF
-        return adv
+        rt.group_launch(self.kernels[g], args, stream, self.grid_blocks)
 
     def run(self, basis, result, status, stream=None):
         """basis u64 [S=1, M, 8, W]; result u64 [1, M, G, W]; status int32 [M, G, *]."""
@@ -114,7 +109,7 @@ class HipGroupSet:
 
     def check_errors(self):
         """Raise if any launch's carry look-back gave up (call after a sync)."""
-        flags = self.counters[:, 1].cpu()
+        flags = self.ctl[:, rt.CTL_ERROR].cpu()
         if int(flags.abs().sum()) != 0:
             bad = [g for g in range(self.n_groups) if int(flags[g]) != 0]
             raise rt.HipRuntimeError(f"carry look-back timed out in groups {bad}; results are invalid")

@@ -53,7 +53,7 @@ class BatchPatternSet:
         programs = []
         for gid, regs in enumerate(self.groups):
             _, _, post, vnm = group_programs(regs, o)
-            programs.append(GroupProgram(gid, post, vnm, regs))
+            programs.append(GroupProgram(gid, post, vnm, regs, lookback=o["lookback"]))
         self.metas = [p.meta() for p in programs]
         d = _set_dir(self.groups, dict(o, form="batch", block=block_threads))
         sources = {}
@@ -97,7 +97,7 @@ class BatchPlan:
     arrays; ``jobs``: list of (set index, buffer index) pairs.  Each pair
     expands to one job per group of the set."""
 
-    def __init__(self, sets, buffers, jobs, device=0, grid_blocks=None):
+    def __init__(self, sets, buffers, jobs, device=0, grid_blocks=None, n_queues=0):
         import torch
 
         self.torch = torch
@@ -160,13 +160,14 @@ class BatchPlan:
             tj = (np.repeat(np.arange(len(entries), dtype=np.int64), [e["T"] for e in entries])
                   if entries else np.zeros(1, dtype=np.int64))
             q["tile_job"] = torch.from_numpy(tj.astype(np.int32)).to(self.dev)
-            q["ctl"] = torch.zeros(4, dtype=torch.int32, device=self.dev)
+            q["ctl"] = rt.new_ctl(self.dev)
             args = rt.BatchArgs()
             args.jobs = q["jobs"].data_ptr()
             args.tile_job = q["tile_job"].data_ptr()
             args.ctl = q["ctl"].data_ptr()
             args.n_jobs = len(entries)
             args.total_tiles = q["total_tiles"]
+            args.n_queues = n_queues
             q["args"] = args
         # one transpose launch for all buffers (a descriptor per buffer)
         td = (rt.TDesc * max(1, self.n_buffers))()
@@ -185,13 +186,12 @@ class BatchPlan:
 
     # -- execution -------------------------------------------------------------
     def _grid(self, ps, total_tiles):
-        waves_per_block = ps.block_threads // 64
+        """Persistent blocks (0 = the runtime's default: as many as fit on the
+        device at once, never more than one wave per tile)."""
         if self.grid_blocks:
-            blocks = self.grid_blocks
-        else:
-            props = self.torch.cuda.get_device_properties(self.dev)
-            blocks = max(1, props.multi_processor_count * 16 // waves_per_block)
-        return max(1, min(blocks, (total_tiles + waves_per_block - 1) // waves_per_block))
+            wpb = ps.block_threads // 64
+            return max(1, min(self.grid_blocks, (total_tiles + wpb - 1) // wpb))
+        return 0
 
     def enqueue_transpose(self, stream):
         rt.transpose_batch(self.bytes.data_ptr(), self.basis.data_ptr(), self.tdesc.data_ptr(), self.n_buffers,
@@ -231,7 +231,7 @@ class BatchPlan:
 
     def check_errors(self):
         for q in self.queues:
-            if int(q["ctl"][1].item()) != 0:
+            if int(q["ctl"][rt.CTL_ERROR].item()) != 0:
                 raise rt.HipRuntimeError("carry look-back timed out in a batched launch; results are invalid")
 
     def results(self, set_index, buffer_index):

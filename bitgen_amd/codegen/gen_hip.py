@@ -48,11 +48,17 @@ def _cname(v):
 class GroupProgram:
     """Analysis + HIP source of one group program."""
 
-    def __init__(self, gid, insts, vnm, regexes=None):
+    def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate"):
+        """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
+        replay when wrong) or "wait" (every carry op waits for its
+        predecessor's status)."""
+        if lookback not in ("speculate", "wait"):
+            raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
         self.gid = gid
         self.insts = insts
         self.vnm = vnm
         self.regexes = regexes or []
+        self.lookback = lookback
         self.kernel_name = f"bg_group_{gid}"
         self._check()
         self._halo()
@@ -151,11 +157,20 @@ class GroupProgram:
         add("  const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;")
         add("  const int64_t w = w0 + lane;")
         add("  const bool inb = (w >= 0) && (w < W);")
-        add("  const bg_tile tc{t, w0, W, status, n_carry, epoch, err};")
+        add("  bg_tile tc = bg_tile_make(t, w0, W, status, n_carry, epoch, err);")
+        if self.n_carry > 64 or self.lookback == "wait":
+            # every op waits for its predecessor (speculation state is kept in
+            # 64-bit masks, so programs with more carry ops always wait)
+            add("  tc.mode = BG_REPLAY;")
         add("  (void)tc;")
         add("  uint64_t bas[8];")
+        # all eight loads in flight at once: clamp the index instead of
+        # predicating each load, then clear the lanes outside the stream
+        add("  const int64_t wc = w < 0 ? 0 : (w >= W ? W - 1 : w);")
         add("#pragma unroll")
-        add("  for (int i = 0; i < 8; i++) bas[i] = inb ? basis[(int64_t)i * W + w] : 0ull;")
+        add("  for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + wc];")
+        add("#pragma unroll")
+        add("  for (int i = 0; i < 8; i++) bas[i] = inb ? bas[i] : 0ull;")
         # declarations (goto must not cross initialisations)
         vars_ = set()
         for inst in self.insts:
@@ -170,6 +185,28 @@ class GroupProgram:
             add(f"  uint64_t {chunk};")
         add("  uint64_t out = 0;")
         carry_index = {id(inst): (k, L) for k, (inst, L) in enumerate(self.carry_ops)}
+        # With carry ops the program after its header runs in a loop of
+        # passes: a speculative pass and, when bg_settle finds an assumed
+        # carry-in was wrong, a replay.  The loop starts before the first skip
+        # test or carry op, and earlier if a variable defined before that
+        # point is assigned again after it; variables defined in the loop are
+        # reset at the top of a pass, so a replay computes exactly what a
+        # single straight run would, and the header stays outside.
+        speculative = 0 < self.n_carry <= 64 and self.lookback == "speculate"
+        body_start = next((k for k, i in enumerate(self.insts)
+                           if i.type in (BsInstType.SKIPZERO, BsInstType.MATCHSTAR, BsInstType.SCANTHRU)),
+                          len(self.insts))
+        while body_start > 0:
+            later = {d for i in self.insts[body_start:] for d in i.defs()}
+            first = next((k for k, i in enumerate(self.insts[:body_start]) if later & set(i.defs())), None)
+            if first is None:
+                break
+            body_start = first
+        body_vars = list(dict.fromkeys(d for i in self.insts[body_start:] for d in i.defs()))
+        self.loop_start = body_start if speculative else None
+        self.carry_bounds = self._carry_bounds(carry_index)
+        if speculative:
+            add("  " + " ".join(f"int bg_infl_{k} = -1;" for k in range(self.n_carry)))
         # values shared by every regex (the basis and the group header defined
         # before the first zero test) are fenced at regex boundaries
         first_skip = next((k for k, i in enumerate(self.insts) if i.type == BsInstType.SKIPZERO), len(self.insts))
@@ -177,7 +214,13 @@ class GroupProgram:
         later = {u for i in self.insts[first_skip:] for u in i.uses()}
         fence_vars = [v for v in dict.fromkeys(header) if v in later]
         open_regions = set()
-        for inst in self.insts:
+        for idx, inst in enumerate(self.insts):
+            if speculative and idx == body_start:
+                add("  for (;;) {")
+                zeros = [f"{_cname(v)} = 0;" for v in body_vars]
+                for i in range(0, len(zeros), 16):
+                    add("  " + " ".join(zeros[i:i + 16]))
+                add("  out = 0;")
             t = inst.type
             if t == BsInstType.STREAMLOAD:
                 add(f"  {_cname(inst.ret)} = bas[{int(inst.stream_index)}];")
@@ -197,8 +240,13 @@ class GroupProgram:
             elif t in (BsInstType.MATCHSTAR, BsInstType.SCANTHRU):
                 k, L = carry_index[id(inst)]
                 fn = "bg_matchstar" if t == BsInstType.MATCHSTAR else "bg_scanthru"
-                add(f"  {_cname(inst.ret)} = {fn}<{L}, S, {k}>({self.val(inst.operands[0])}, "
-                    f"{self.val(inst.operands[1])}, tc);")
+                if speculative:
+                    add(f"  {_cname(inst.ret)} = {fn}<{L}, S, {k}>({self.val(inst.operands[0])}, "
+                        f"{self.val(inst.operands[1])}, tc, {self._bound_expr(k)}, bg_infl_{k});")
+                    add("  if (tc.abort) goto bg_pass_end;")
+                else:
+                    add(f"  {{ int bg_unused = -1; {_cname(inst.ret)} = {fn}<{L}, S, {k}>("
+                        f"{self.val(inst.operands[0])}, {self.val(inst.operands[1])}, tc, -1, bg_unused); }}")
             elif t == BsInstType.SKIPZERO:
                 add(f"  if (bg_wave_zero({self.val(inst.operands[0])})) goto L{inst.label};")
                 open_regions.add(inst.label)
@@ -215,11 +263,64 @@ class GroupProgram:
                 pass
             else:
                 raise HipCodegenError(f"unsupported instruction {inst}")
+        if speculative:
+            add("bg_pass_end:;")
+            add(f"  if (bg_settle<{self.n_carry}>(tc)) break;")
+            add("  }")
         add("  const int lo = (t == 0) ? 0 : HL;")
         add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
         add("  if (inb && lane >= lo && lane < hi) result[w] = out;")
         add("}")
         return "\n".join(lines) + "\n"
+
+    def _carry_bounds(self, carry_index):
+        """For each carry op k: {earlier carry op j: lanes of right advance
+        between j's result and k's operands}, from a program-order walk.  A
+        variable's map only grows (reassignments and skip-region joins merge),
+        and a carry op's result depends on itself alone: its reported bound
+        already covers its operands'."""
+        deps = {}
+        bounds = {}
+
+        def merge(into, other, shift=0):
+            for j, b in other.items():
+                into[j] = max(into.get(j, 0), b + shift)
+
+        for inst in self.insts:
+            t = inst.type
+            if t in (BsInstType.SKIPZERO, BsInstType.LABEL, BsInstType.STREAMSTORE, BsInstType.GRAPHBREAK):
+                continue
+            if t == BsInstType.STREAMLOAD:
+                deps.setdefault(inst.ret, {})
+                continue
+            ops = {}
+            for u in inst.uses():
+                merge(ops, deps.get(u, {}))
+            if t in (BsInstType.MATCHSTAR, BsInstType.SCANTHRU):
+                k = carry_index[id(inst)][0]
+                bounds[k] = ops
+                res = {k: 0}
+            elif t == BsInstType.ADVANCE and not inst.is_left:
+                res = {}
+                merge(res, ops, inst.amount)
+            else:
+                res = ops
+            cur = deps.setdefault(inst.ret, {})
+            merge(cur, res)
+        return bounds
+
+    def _bound_expr(self, k):
+        """C expression of carry op k's operand bound (see bg_wave_add)."""
+        terms = []
+        for j, bits in sorted(self.carry_bounds.get(k, {}).items()):
+            lanes = math.ceil(bits / WAVE)
+            terms.append(f"bg_infl_{j}" + (f" + {lanes}" if lanes else ""))
+        if not terms:
+            return "-1"
+        e = terms[0]
+        for t in terms[1:]:
+            e = f"bg_imax({e}, {t})"
+        return e
 
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
@@ -227,13 +328,18 @@ class GroupProgram:
             '#include "bitgen_device.h"',
             self.device_function(),
             f'extern "C" __global__ __launch_bounds__(256) void {self.kernel_name}(bg_group_args_t a) {{',
-            "  // persistent: each wave takes tiles until none is left, so the",
-            "  // program's executed code stays in the instruction cache",
-            "  while (true) {",
-            "    const uint32_t t = bg_acquire_tile(a.counter, a.tile_base);",
-            "    if (t >= a.n_tiles) return;",
-            f"    bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, a.epoch, a.n_carry, a.counter + 1);",
+            "  // persistent: each wave takes tiles from its block's queue until the",
+            "  // queue is exhausted",
+            "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles);",
+            "  uint32_t t;",
+            "  bool more = bg_queue_take(Q, t);",
+            "  while (more) {",
+            "    const uint32_t next = bg_queue_issue(Q);  // next tile, taken ahead",
+            f"    bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
+            "a.ctl + BG_CTL_ERROR);",
+            "    more = bg_queue_finish(Q, next, t);",
             "  }",
+            "  bg_queue_close(Q);",
             "}",
             "",
         ])
@@ -265,14 +371,18 @@ def batch_source(program, block_threads=256):
         '#include "bitgen_device.h"',
         program.device_function(),
         f'extern "C" __global__ __launch_bounds__({block_threads}) void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
-        "  const uint32_t epoch = a.ctl[2];",
-        "  while (true) {",
-        "    const uint32_t tg = bg_acquire_tile(a.ctl, 0);",
-        "    if (tg >= a.total_tiles) return;",
+        "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles);",
+        "  uint32_t tg;",
+        "  bool more = bg_queue_take(Q, tg);",
+        "  while (more) {",
+        "    const uint32_t next = bg_queue_issue(Q);  // next tile, taken ahead",
         "    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
         "    const bg_job_t J = a.jobs[j];",
-        f"    bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, epoch, J.n_carry, a.ctl + 1);",
+        f"    bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, Q.epoch, J.n_carry, "
+        "a.ctl + BG_CTL_ERROR);",
+        "    more = bg_queue_finish(Q, next, tg);",
         "  }",
+        "  bg_queue_close(Q);",
         "}",
         "",
     ])

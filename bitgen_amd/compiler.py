@@ -30,6 +30,10 @@ DEFAULT_OPTIONS = {
     "pass_short_circuit_start": 0,
     "pass_short_circuit_interval": 1,
     "pass_graph_break": -1,
+    # carry look-back of the hip kernels: "speculate" (assume carry-in 0,
+    # resolve once per tile, replay when wrong) or "wait" (every carry op
+    # waits for its predecessor tile)
+    "lookback": "speculate",
 }
 
 
@@ -126,7 +130,7 @@ def compile_pattern_set(groups, options=None, workers=None):
     programs, sources = [], {}
     for gid, regs in enumerate(groups):
         _, _, post, post_vnm = group_programs(regs, o)
-        prog = GroupProgram(gid, post, post_vnm, regs)
+        prog = GroupProgram(gid, post, post_vnm, regs, lookback=o["lookback"])
         programs.append(prog)
         path = os.path.join(d, f"kernel_{gid}.hip")
         src = prog.source()

@@ -44,22 +44,14 @@ struct bg_module_s {
 
 struct bg_kernel_s {
   hipFunction_t fn;
+  uint32_t occ_threads;  // block size of the cached occupancy (0 = none)
+  int32_t occ_blocks;    // blocks per CU at that block size
 };
 
 struct bg_graph_s {
   hipGraph_t graph;
   hipGraphExec_t exec;
 };
-
-// Resets the tile counter and advances the epoch (1 .. 2^28-2) of a batched
-// launch on the device, so a captured graph replays with fresh status tags.
-__global__ void bg_batch_prologue_kernel(uint32_t *ctl) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    ctl[0] = 0;
-    const uint32_t e = ctl[2];
-    ctl[2] = (e >= (1u << 28) - 2) ? 1u : e + 1u;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Transpose: thread = one 64-byte block (one word of each basis stream) of one
@@ -263,29 +255,73 @@ int bg_kernel_info(bg_kernel_t k, bg_kernel_info_t *info) {
   return 0;
 }
 
-int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t *counter_advance,
-                    void *stream) {
-  if (!k || !args) return set_err(BG_EINVAL, "bg_group_launch: null argument");
-  if (!args->basis || !args->result || !args->status || !args->counter)
-    return set_err(BG_EINVAL, "bg_group_launch: null buffer");
-  if (args->n_tiles == 0) {
-    if (counter_advance) *counter_advance = 0;
-    return 0;
+namespace {
+
+// Blocks of `threads` threads that fit on one CU (cached per kernel).
+int occupancy(bg_kernel_t k, uint32_t threads, int32_t *blocks_per_cu) {
+  if (k->occ_threads != threads) {
+    int n = 0;
+    BG_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&n, k->fn, (int)threads, 0),
+           "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    k->occ_blocks = n > 0 ? n : 1;
+    k->occ_threads = threads;
   }
-  if (args->epoch == 0 || args->epoch >= (1u << 28))
-    return set_err(BG_EINVAL, "bg_group_launch: epoch must be in 1 .. 2^28-1");
-  // persistent waves: enough to fill every CU, never more than the tiles
-  const uint32_t waves_per_block = 4;
-  const uint32_t max_blocks = (uint32_t)g_cu_count * 8;
-  uint32_t blocks = (args->n_tiles + waves_per_block - 1) / waves_per_block;
-  if (blocks > max_blocks) blocks = max_blocks;
+  *blocks_per_cu = k->occ_blocks;
+  return 0;
+}
+
+// Persistent grid: the blocks that fit on the device at once (or the
+// caller's count), never more than one wave per tile.
+int persistent_grid(bg_kernel_t k, uint32_t threads, uint32_t n_tiles, uint32_t requested, uint32_t *blocks) {
+  const uint32_t wpb = threads / 64;
+  uint32_t b = requested;
+  if (b == 0) {
+    int32_t per_cu = 1;
+    int rc = occupancy(k, threads, &per_cu);
+    if (rc) return rc;
+    b = (uint32_t)per_cu * (uint32_t)g_cu_count;
+  }
+  const uint32_t need = (n_tiles + wpb - 1) / wpb;
+  if (b > need) b = need;
+  *blocks = b ? b : 1;
+  return 0;
+}
+
+uint32_t clamp_queues(uint32_t nq, uint32_t blocks) {
+  if (nq == 0) nq = 8;
+  if (nq > BG_MAX_QUEUES) nq = BG_MAX_QUEUES;
+  if (nq > blocks) nq = blocks;
+  return nq ? nq : 1;
+}
+
+}  // namespace
+
+int bg_kernel_occupancy(bg_kernel_t k, uint32_t block_threads, int32_t *blocks_per_cu, int32_t *cu_count) {
+  if (!k || !blocks_per_cu) return set_err(BG_EINVAL, "bg_kernel_occupancy: null argument");
+  if (block_threads == 0 || block_threads % 64 || block_threads > 1024)
+    return set_err(BG_EINVAL, "bg_kernel_occupancy: block_threads must be a multiple of 64, at most 1024");
+  int rc = occupancy(k, block_threads, blocks_per_cu);
+  if (rc) return rc;
+  if (cu_count) *cu_count = g_cu_count;
+  return 0;
+}
+
+uint32_t bg_ctl_words(void) { return BG_CTL_WORDS; }
+
+int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t grid_blocks, void *stream) {
+  if (!k || !args) return set_err(BG_EINVAL, "bg_group_launch: null argument");
+  if (!args->basis || !args->result || !args->status || !args->ctl)
+    return set_err(BG_EINVAL, "bg_group_launch: null buffer");
+  if (args->n_tiles == 0) return 0;
+  const uint32_t threads = 256;
+  uint32_t blocks = 0;
+  int rc = persistent_grid(k, threads, args->n_tiles, grid_blocks, &blocks);
+  if (rc) return rc;
   bg_group_args_t a = *args;
+  a.n_queues = clamp_queues(a.n_queues, blocks);
   void *params[] = {&a};
-  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, 64 * waves_per_block, 1, 1, 0, S(stream), params,
-                               nullptr),
+  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, threads, 1, 1, 0, S(stream), params, nullptr),
          "hipModuleLaunchKernel");
-  // every tile is taken once and every wave makes one final, failing take
-  if (counter_advance) *counter_advance = args->n_tiles + blocks * waves_per_block;
   return 0;
 }
 
@@ -318,13 +354,14 @@ int bg_batch_launch(bg_kernel_t k, const bg_batch_args_t *args, uint32_t grid_bl
     return set_err(BG_EINVAL, "bg_batch_launch: null argument");
   if (block_threads == 0 || block_threads % 64 || block_threads > 1024)
     return set_err(BG_EINVAL, "bg_batch_launch: block_threads must be a multiple of 64, at most 1024");
-  if (grid_blocks == 0) return set_err(BG_EINVAL, "bg_batch_launch: grid_blocks must be positive");
-  hipLaunchKernelGGL(bg_batch_prologue_kernel, dim3(1), dim3(64), 0, S(stream), args->ctl);
-  BG_HIP(hipGetLastError(), "batch prologue launch");
   if (args->total_tiles == 0) return 0;
+  uint32_t blocks = 0;
+  int rc = persistent_grid(k, block_threads, args->total_tiles, grid_blocks, &blocks);
+  if (rc) return rc;
   bg_batch_args_t a = *args;
+  a.n_queues = clamp_queues(a.n_queues, blocks);
   void *params[] = {&a};
-  BG_HIP(hipModuleLaunchKernel(k->fn, grid_blocks, 1, 1, block_threads, 1, 1, 0, S(stream), params, nullptr),
+  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, block_threads, 1, 1, 0, S(stream), params, nullptr),
          "hipModuleLaunchKernel(batch)");
   return 0;
 }

@@ -16,7 +16,10 @@ LIB_PATH = os.path.join(PKG, "libbitgen_hip.so")
 ARCH = os.environ.get("BITGEN_HIP_ARCH", "gfx950")
 
 # Packed-FP32 instruction forms are disabled for every HIP translation unit.
+# The atomic optimizer is off: it rewrites the tile-queue atomic so that its
+# result is read right after it is sent, which defeats taking tiles ahead.
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
              "-std=c++17", "-Wno-unused-command-line-argument"]
 
 

@@ -99,7 +99,7 @@ class MatchServer:
         self._thread = None
         props = torch.cuda.get_device_properties(self.dev)
         self.grid_waves = props.multi_processor_count * grid_waves_per_cu
-        self.ctl = {}  # (set, group, slot) -> ctl tensor (tile counter, error flag, epoch)
+        self.ctl = {}  # (set, group, slot) -> control block (tile queues, error flag, epoch)
 
     # -- client side -------------------------------------------------------------
     def submit(self, set_id, data):
@@ -221,7 +221,7 @@ class MatchServer:
                 tj = torch.from_numpy(tj_host).pin_memory().to(self.dev, non_blocking=True)
                 key = (sid, g, id(slot))
                 if key not in self.ctl:
-                    self.ctl[key] = torch.zeros(4, dtype=torch.int32, device=self.dev)
+                    self.ctl[key] = rt.new_ctl(self.dev)
                 ctl = self.ctl[key]
                 args = rt.BatchArgs()
                 args.jobs = jt.data_ptr()
@@ -258,7 +258,7 @@ class MatchServer:
                 self.completed.append(job)
                 job.future.set_result(res)
             for (sid, g, sl), ctl in self.ctl.items():
-                if sl == id(slot) and int(ctl[1].item()) != 0:
+                if sl == id(slot) and int(ctl[rt.CTL_ERROR].item()) != 0:
                     raise rt.HipRuntimeError("carry look-back timed out in a served batch")
             slot.jobs = slot.keep = slot.host_out = slot.out_rows = slot.event = None
             progressed = True

@@ -137,12 +137,41 @@ struct bg_tile {
   uint32_t n_carry;   // carry operations of the program
   uint32_t epoch;     // launch epoch (status words of older launches are ignored)
   uint32_t *error;    // set to nonzero when a look-back wait gives up
+  // Carry-in handling, wave-uniform (see bg_wave_add and bg_settle).
+  uint32_t mode;      // BG_SPECULATE or BG_REPLAY
+  uint32_t k_exact;   // replay: carry-ins of ops < k_exact are in `cin`
+  uint32_t abort;     // speculate: an op could not publish; the pass stops
+  uint64_t cin;       // replay: carry-in bits of ops < k_exact
+  uint64_t agg;       // speculate: ops that published an aggregate
+  uint64_t out1;      // speculate: their carry-out for carry-in 1
 };
+
+#define BG_SPECULATE 0u
+#define BG_REPLAY 1u
+
+__device__ __forceinline__ bg_tile bg_tile_make(uint32_t t, int64_t w0, int64_t W, uint32_t *status,
+                                                uint32_t n_carry, uint32_t epoch, uint32_t *error) {
+  bg_tile tc;
+  tc.t = t;
+  tc.w0 = w0;
+  tc.W = W;
+  tc.status = status;
+  tc.n_carry = n_carry;
+  tc.epoch = epoch;
+  tc.error = error;
+  tc.mode = BG_SPECULATE;
+  tc.k_exact = 0;
+  tc.abort = 0;
+  tc.cin = 0;
+  tc.agg = 0;
+  tc.out1 = 0;
+  return tc;
+}
 
 // Upper bound on look-back polls (each sleeps ~64 cycles): far above any
 // legitimate wait, low enough that a fault never hangs the device.
 #ifndef BG_SPIN_LIMIT
-#define BG_SPIN_LIMIT (1u << 24)
+#define BG_SPIN_LIMIT (1u << 21)
 #endif
 
 // Status word: bits 0..1 value, bits 2..3 kind, bits 4..31 epoch.
@@ -161,6 +190,43 @@ __device__ __forceinline__ uint32_t bg_st_load(const uint32_t *p) {
   return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// First lane >= x whose bit in `notp` is set (64 if none); x in [0, 64].
+__device__ __forceinline__ int bg_imax(int a, int b) { return a > b ? a : b; }
+
+__device__ __forceinline__ int bg_first_from(uint64_t notp, int x) {
+  if (x >= 64) return 64;
+  const uint64_t m = notp & (~0ull << x);
+  return m ? (int)__builtin_ctzll(m) : 64;
+}
+
+// Look-back for the carry into op K of tile t: walk predecessors from t - 1
+// until an inclusive status is found (aggregates propagate; tile 0 is always
+// inclusive).  st0 is tile t - 1's status word if already loaded (else 0).
+// The wait is bounded so that a missing predecessor shows up as an error
+// flag instead of a hung wave.
+__device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, uint32_t st) {
+  int64_t j = (int64_t)tc.t - 1;
+  uint32_t spins = 0;
+  bool have = st != 0;
+  while (true) {
+    if (!have) st = bg_st_load(tc.status + (size_t)j * tc.n_carry + K);
+    have = false;
+    const uint32_t kind = (st >> 2) & 3u;
+    if ((st >> 4) != (tc.epoch & 0x0FFFFFFFu) || kind == 0) {
+      if (++spins > BG_SPIN_LIMIT) {
+        if (bg_lane() == 0) __hip_atomic_fetch_or(tc.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    if (kind == BG_ST_INC) return st & 1u;
+    // aggregate: the whole predecessor segment propagates; look further back
+    j -= 1;
+    if (j < 0) return 0;
+  }
+}
+
 // Wave-wide addition a + b of the tile's 64-word number with the carry into
 // lane L taken from the predecessor tile (decoupled look-back); returns this
 // lane's sum word.  K = index of the carry operation in the program.
@@ -170,8 +236,25 @@ __device__ __forceinline__ uint32_t bg_st_load(const uint32_t *p) {
 // is bit l of ((G | P) + G + (cin << L)) ^ P: a ripple adder whose per-lane
 // generate is G and propagate is P.  The carry handed to the next tile is the
 // carry into lane L + S.
+//
+// Three modes (wave-uniform):
+//  * tile 0: carry-in 0, exact; publishes at once.
+//  * BG_SPECULATE (first pass of tile t >= 1): the carry-in is assumed 0 and
+//    nothing waits.  `bnd` bounds the lanes of this op's operands that
+//    earlier assumptions may have changed (-1: none; the generator takes the
+//    maximum over the carry ops the operands depend on, plus the lanes right
+//    advances can move values up).  The carry into lane L + S cannot depend
+//    on any assumption when a non-propagating lane lies between bnd and
+//    L + S (or bnd < L); the status is then published at once.  Otherwise
+//    the pass aborts at this op.  `infl` returns the bound for this op's
+//    result.  bg_settle then resolves every carry-in with one batch of
+//    look-back loads and replays the tile if an assumption was wrong or the
+//    pass aborted.
+//  * BG_REPLAY: ops < k_exact take their resolved carry-in and re-publish the
+//    same status; ops >= k_exact wait for their predecessor (publish, look
+//    back, publish the inclusive value).
 template <int L, int S, int K>
-__device__ __forceinline__ uint64_t bg_wave_add(uint64_t a, uint64_t b, const bg_tile &tc) {
+__device__ __forceinline__ uint64_t bg_wave_add(uint64_t a, uint64_t b, bg_tile &tc, int bnd, int &infl) {
   const int lane = bg_lane();
   const int l0 = (tc.t == 0) ? 0 : L;  // tile 0 is exact from lane 0
   const uint64_t s = a + b;
@@ -196,72 +279,207 @@ __device__ __forceinline__ uint64_t bg_wave_add(uint64_t a, uint64_t b, const bg
 
   uint32_t *my = tc.status + (size_t)tc.t * tc.n_carry + K;
   uint32_t cin = 0;
+  constexpr bool kSpeculate = K < 64;  // speculation state is kept in 64-bit masks
   if (tc.t == 0) {
     const uint32_t out = carry_out(0);
     if (lane == 0) bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_INC, out));
+  } else if (!kSpeculate || tc.mode == BG_REPLAY) {
+    if (kSpeculate && (uint32_t)K < tc.k_exact) {
+      cin = (uint32_t)(tc.cin >> (K & 63)) & 1u;
+      if (lane == 0) bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_INC, carry_out(cin)));
+    } else {
+      const uint32_t out0 = carry_out(0), out1 = carry_out(1);
+      if (lane == 0) bg_st_store(my, out0 == out1 ? bg_st_pack(tc.epoch, BG_ST_INC, out0)
+                                                  : bg_st_pack(tc.epoch, BG_ST_AGG, 0));
+      cin = bg_lookback(tc, K, 0);
+      if (out0 != out1 && lane == 0) bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_INC, cin));
+    }
   } else {
+    // speculate: operand lanes <= b may differ from their exact values
+    const int b = bnd > 63 ? 63 : bnd;
+    const uint64_t notp = ~P & keep;
+    if (b >= l0 && bg_first_from(notp, b + 1) >= pass) {
+      tc.abort = 1;
+      tc.k_exact = K;
+      return 0;
+    }
     const uint32_t out0 = carry_out(0), out1 = carry_out(1);
     if (out0 == out1) {
       if (lane == 0) bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_INC, out0));
-    } else if (lane == 0) {
-      bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_AGG, 0));
+    } else {
+      if (lane == 0) bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_AGG, 0));
+      tc.agg |= 1ull << (K & 63);
+      tc.out1 |= (uint64_t)out1 << (K & 63);
     }
-    // look back for the carry into lane L; the wait is bounded so that a
-    // missing predecessor shows up as an error flag instead of a hung wave
-    int64_t j = (int64_t)tc.t - 1;
-    uint32_t spins = 0;
-    while (true) {
-      const uint32_t st = bg_st_load(tc.status + (size_t)j * tc.n_carry + K);
-      const uint32_t kind = (st >> 2) & 3u;
-      if ((st >> 4) != (tc.epoch & 0x0FFFFFFFu) || kind == 0) {
-        if (++spins > BG_SPIN_LIMIT) {
-          if (lane == 0) __hip_atomic_fetch_or(tc.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          cin = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        continue;
-      }
-      if (kind == BG_ST_INC) {
-        cin = st & 1u;
-        break;
-      }
-      // aggregate: the whole predecessor segment propagates; look further back
-      j -= 1;
-      if (j < 0) {
-        cin = 0;
-        break;
-      }
+    // lanes of this op's result that differ if an assumption was wrong: a
+    // carry-in of 1 changes lanes L through the first non-propagating lane,
+    // and changed operand lanes <= b carry on through the lanes that
+    // propagate
+    int hi = bg_first_from(notp, l0);
+    if (b >= l0) {
+      const int zb = bg_first_from(notp, b + 1);
+      hi = hi > zb ? hi : zb;
     }
-    if (out0 != out1 && lane == 0) bg_st_store(my, bg_st_pack(tc.epoch, BG_ST_INC, cin));
+    hi = hi > 63 ? 63 : hi;
+    infl = hi > b ? hi : b;
   }
   const uint64_t cm = (GP + G + ((uint64_t)cin << l0)) ^ P;
   const uint64_t carry = (cm >> lane) & 1ull;
   return s + carry;
 }
 
+// End of a pass.  Returns true when the pass's values are exact.  After a
+// speculative pass: resolves the carry-ins of the ops the pass reached (one
+// batch of loads, then the waiting look-back only where a predecessor had not
+// published yet), publishes the inclusive status of the ops that published an
+// aggregate, and switches to a replay if an assumed carry-in was 1 or the
+// pass aborted.  NC = carry operations of the program (<= 64).
+template <int NC>
+__device__ __forceinline__ bool bg_settle(bg_tile &tc) {
+  static_assert(NC >= 1 && NC <= 64, "programs with more than 64 carry operations start in replay mode");
+  if (tc.t == 0 || tc.mode == BG_REPLAY) return true;
+  const uint32_t n = tc.abort ? tc.k_exact : (uint32_t)NC;
+  const uint32_t *prev = tc.status + (size_t)(tc.t - 1) * tc.n_carry;
+  uint32_t st[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) st[k] = ((uint32_t)k < n) ? bg_st_load(prev + k) : 0u;
+  uint64_t cin = 0;
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    if ((uint32_t)k >= n) break;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)st[k]);
+    uint32_t c;
+    if ((w >> 4) == (tc.epoch & 0x0FFFFFFFu) && ((w >> 2) & 3u) == BG_ST_INC)
+      c = w & 1u;
+    else
+      c = bg_lookback(tc, (uint32_t)k, w);
+    cin |= (uint64_t)c << k;
+  }
+  if (tc.agg) {
+    uint64_t a = tc.agg;
+    while (a) {
+      const int k = __builtin_ctzll(a);
+      a &= a - 1;
+      const uint32_t c = (uint32_t)(cin >> k) & 1u;
+      const uint32_t o1 = (uint32_t)(tc.out1 >> k) & 1u;
+      if (bg_lane() == 0)
+        bg_st_store(tc.status + (size_t)tc.t * tc.n_carry + k, bg_st_pack(tc.epoch, BG_ST_INC, c ? o1 : 1u - o1));
+    }
+  }
+  if (!tc.abort && cin == 0) return true;
+  tc.mode = BG_REPLAY;
+  tc.k_exact = n;
+  tc.cin = cin;
+  tc.abort = 0;
+  return false;
+}
+
 // MatchStar(M, C) = (((M & C) + C) ^ C) | M
 template <int L, int S, int K>
-__device__ __forceinline__ uint64_t bg_matchstar(uint64_t m, uint64_t c, const bg_tile &tc) {
-  const uint64_t sum = bg_wave_add<L, S, K>(m & c, c, tc);
+__device__ __forceinline__ uint64_t bg_matchstar(uint64_t m, uint64_t c, bg_tile &tc, int bnd, int &infl) {
+  const uint64_t sum = bg_wave_add<L, S, K>(m & c, c, tc, bnd, infl);
   return (sum ^ c) | m;
 }
 
 // ScanThru(M, C) = (M + C) & ~C
 template <int L, int S, int K>
-__device__ __forceinline__ uint64_t bg_scanthru(uint64_t m, uint64_t c, const bg_tile &tc) {
-  const uint64_t sum = bg_wave_add<L, S, K>(m, c, tc);
+__device__ __forceinline__ uint64_t bg_scanthru(uint64_t m, uint64_t c, bg_tile &tc, int bnd, int &infl) {
+  const uint64_t sum = bg_wave_add<L, S, K>(m, c, tc, bnd, infl);
   return sum & ~c;
 }
 
-// ---- tile acquisition --------------------------------------------------------
-// Every wave takes the next tile index from a monotonically increasing counter
-// (counter[0]); args.tile_base is the counter value at launch, so waves run
-// tiles in acquisition order and a tile's predecessors always started earlier
-// (required by the look-back spin).
-__device__ __forceinline__ uint32_t bg_acquire_tile(uint32_t *counter, uint32_t base) {
-  uint32_t t = 0;
-  if (bg_lane() == 0) t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-  return t - base;
+// ---- tile queues -------------------------------------------------------------
+// A launch's tiles are dealt round-robin over nq queues (tile t belongs to
+// queue t % nq); block b serves queue b % nq, one atomic add per tile on the
+// queue's own counter line.  Waves of a queue take its tiles in increasing
+// order, and the look-back only waits for smaller tiles, so every wait ends
+// once the first nq blocks are resident.  The control block layout
+// (BG_CTL_*) is in bitgen_hip.h.
+struct bg_queue_lds;
+
+struct bg_queue {
+  bg_queue_lds *lds;  // per-block state (LDS)
+  uint32_t *ctl;
+  uint32_t q;       // this block's queue
+  uint32_t nq;      // queues of the launch
+  uint32_t n;       // tiles of the launch
+  uint32_t epoch;   // status tag of the launch
+};
+
+__device__ __forceinline__ uint32_t bg_ctl_load(uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-block state of the queue protocol (LDS).
+struct bg_queue_lds {
+  uint32_t epoch;
+  uint32_t exits;
+};
+
+// Device-scope accesses to one shared word serialise, so each block makes
+// one of each: a single epoch read at the start (shared through LDS) and a
+// single done-increment when its last wave leaves.
+__device__ __forceinline__ bg_queue bg_queue_open(uint32_t *ctl, uint32_t nq, uint32_t n_tiles) {
+  __shared__ bg_queue_lds lds;
+  bg_queue Q;
+  Q.lds = &lds;
+  Q.ctl = ctl;
+  // the launch functions clamp nq to 1 .. gridDim.x; clamp again so that every
+  // queue is served by at least one block whatever the caller passed
+  Q.nq = (nq == 0) ? 1u : (nq < gridDim.x ? nq : gridDim.x);
+  Q.q = blockIdx.x % Q.nq;
+  Q.n = n_tiles;
+  if (threadIdx.x == 0) {
+    lds.epoch = bg_ctl_load(ctl + BG_CTL_EPOCH) & 0x0FFFFFFFu;
+    lds.exits = 0;
+  }
+  __syncthreads();
+  Q.epoch = (uint32_t)__builtin_amdgcn_readfirstlane((int)lds.epoch);
+  return Q;
+}
+
+// Taking a tile is split in two so that a wave can ask for its next tile
+// before running the current one (the atomic's latency then overlaps the
+// tile's own loads): bg_queue_issue sends the atomic add (the old value
+// lands in lane 0), bg_queue_finish turns it into a tile index and returns
+// false when the queue is exhausted.  Each wave makes exactly one failing
+// take.  A wave holds at most its current tile and the next, larger one, so
+// the smallest unfinished tile is always being run and every look-back wait
+// ends.
+__device__ __forceinline__ uint32_t bg_queue_issue(const bg_queue &Q) {
+  uint32_t k = 0;
+  if (bg_lane() == 0)
+    k = __hip_atomic_fetch_add(Q.ctl + BG_CTL_COUNTER(Q.q), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return k;
+}
+
+__device__ __forceinline__ bool bg_queue_finish(const bg_queue &Q, uint32_t k, uint32_t &t) {
+  k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+  const uint64_t tt = (uint64_t)k * Q.nq + Q.q;
+  t = (uint32_t)tt;
+  return tt < Q.n;
+}
+
+__device__ __forceinline__ bool bg_queue_take(const bg_queue &Q, uint32_t &t) {
+  return bg_queue_finish(Q, bg_queue_issue(Q), t);
+}
+
+// Called once by every wave after its last take.  The last wave of a block
+// reports the block; the last block of a queue resets the queue's counters;
+// the last queue to finish advances the epoch.  Every block read the epoch
+// before it reported, so the advance is seen only by the next launch.
+__device__ __forceinline__ void bg_queue_close(const bg_queue &Q) {
+  if (bg_lane() != 0) return;
+  const uint32_t wpb = blockDim.x / BG_WAVE;
+  const uint32_t e = __hip_atomic_fetch_add(&Q.lds->exits, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  if (e + 1 != wpb) return;
+  const uint32_t blocks_q = (gridDim.x - Q.q + Q.nq - 1) / Q.nq;
+  const uint32_t d = __hip_atomic_fetch_add(Q.ctl + BG_CTL_DONE(Q.q), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (d + 1 != blocks_q) return;
+  __hip_atomic_store(Q.ctl + BG_CTL_COUNTER(Q.q), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(Q.ctl + BG_CTL_DONE(Q.q), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t qd = __hip_atomic_fetch_add(Q.ctl + BG_CTL_QDONE, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  if (qd + 1 != Q.nq) return;
+  __hip_atomic_store(Q.ctl + BG_CTL_QDONE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(Q.ctl + BG_CTL_EPOCH, (Q.epoch + 1u) & 0x0FFFFFFFu, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -24,20 +24,45 @@ extern "C" {
 typedef struct bg_module_s *bg_module_t;
 typedef struct bg_kernel_s *bg_kernel_t;
 
+/* Tile queues.  A launch's tiles are dealt round-robin over n_queues
+ * queues (tile t belongs to queue t % n_queues); block b takes tiles from
+ * queue b % n_queues, each wave one tile at a time with one atomic add on its
+ * queue's counter.  Within a queue tiles are taken in increasing order, and
+ * a wave only ever waits (carry look-back) for smaller tiles, so no wave waits
+ * on a tile that nobody can take once the first n_queues blocks are
+ * resident.  One counter per queue spreads the atomics over n_queues
+ * addresses (a single device-wide counter serialises them).
+ *
+ * Control block ("ctl"): BG_CTL_WORDS u32 words, zeroed once before first
+ * use and then owned by one launch at a time (launches that share a ctl must
+ * be ordered on one stream).  Each word below sits on its own 256-byte line.
+ * The last wave of each queue resets that queue's counters; the last queue to
+ * finish advances the epoch, so consecutive launches (and graph replays)
+ * need no host-side reset. */
+#define BG_MAX_QUEUES 32
+#define BG_CTL_STRIDE 64                                   /* u32 words   */
+#define BG_CTL_COUNTER(q) ((q) * BG_CTL_STRIDE)            /* tile counter */
+#define BG_CTL_DONE(q) ((BG_MAX_QUEUES + (q)) * BG_CTL_STRIDE) /* waves out */
+#define BG_CTL_QDONE (2 * BG_MAX_QUEUES * BG_CTL_STRIDE)   /* queues out   */
+#define BG_CTL_ERROR (BG_CTL_QDONE + BG_CTL_STRIDE)        /* sticky error */
+#define BG_CTL_EPOCH (BG_CTL_ERROR + BG_CTL_STRIDE)        /* launch epoch */
+#define BG_CTL_WORDS (BG_CTL_EPOCH + BG_CTL_STRIDE)
+
 /* Launch descriptor of a generated group kernel (one group over one buffer).
  * Field order is part of the ABI: generated kernels take it by value. */
 typedef struct bg_group_args {
   const uint64_t *basis; /* [8][n_words] basis streams, LSB-first words      */
   uint64_t *result;      /* [n_words] match-end stream of the group          */
-  uint32_t *status;      /* [n_tiles][n_carry] carry look-back status words  */
-  uint32_t *counter;     /* [0] tile counter (monotonic, never reset),
-                            [1] error flag, set nonzero if a carry look-back
-                            wait gives up (results are then invalid)         */
+  uint32_t *status;      /* [n_tiles][n_carry] carry look-back status words,
+                            zeroed before the first launch with this ctl     */
+  uint32_t *ctl;         /* BG_CTL_WORDS words; ctl[BG_CTL_ERROR] is set
+                            nonzero if a carry look-back wait gives up
+                            (results are then invalid; the caller clears it) */
   uint64_t n_words;      /* words per stream                                 */
   uint32_t n_tiles;      /* tiles covering n_words                           */
-  uint32_t tile_base;    /* value of *counter when this launch starts        */
-  uint32_t epoch;        /* launch epoch, 1 .. 2^28-1, distinct per launch   */
+  uint32_t n_queues;     /* 0 = default; clamped to 1 .. min(32, blocks)     */
   uint32_t n_carry;      /* carry operations in the program (>= 1 slot)      */
+  uint32_t reserved;
 } bg_group_args_t;
 
 /* One (group, buffer) job of a batched launch.  Field order is ABI. */
@@ -59,9 +84,11 @@ typedef struct bg_job {
 typedef struct bg_batch_args {
   const bg_job_t *jobs;
   const uint32_t *tile_job; /* [total_tiles] job index of each global tile */
-  uint32_t *ctl;            /* [0] tile counter, [1] error flag, [2] epoch  */
+  uint32_t *ctl;            /* BG_CTL_WORDS words (see above)              */
   uint32_t n_jobs;
   uint32_t total_tiles;
+  uint32_t n_queues;        /* 0 = default; clamped like bg_group_args     */
+  uint32_t reserved;
 } bg_batch_args_t;
 
 typedef struct bg_graph_s *bg_graph_t;
@@ -120,14 +147,22 @@ int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out);
 /* Resource usage (cuda_kernel.py:296-398 prints the CUDA equivalents). */
 int bg_kernel_info(bg_kernel_t kernel, bg_kernel_info_t *info);
 
+/* Blocks of block_threads threads of `kernel` that fit on one CU at once
+ * (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor), and the device's CU
+ * count; the default persistent grid is their product. */
+int bg_kernel_occupancy(bg_kernel_t kernel, uint32_t block_threads, int32_t *blocks_per_cu,
+                        int32_t *cu_count);
+
+/* Words of a control block (BG_CTL_WORDS), for callers without this header. */
+uint32_t bg_ctl_words(void);
+
 /* Launch one generated group kernel over one buffer: persistent blocks of
- * 256 threads (at most 8 per CU, at most one wave per tile) that take tiles
- * from the counter until none is left.  Replaces CUDA_Kernel.run
- * (cuda_kernel.py:411-427), asynchronously: the caller synchronises.  The
- * caller must advance args->tile_base by *counter_advance before the next
- * launch sharing the counter. */
-int bg_group_launch(bg_kernel_t kernel, const bg_group_args_t *args,
-                    uint32_t *counter_advance, void *stream);
+ * 256 threads that take tiles from the control block's queues until none is
+ * left (grid_blocks = 0: as many blocks as fit on the device at once, never
+ * more than one wave per tile).  Replaces CUDA_Kernel.run
+ * (cuda_kernel.py:411-427), asynchronously: the caller synchronises. */
+int bg_group_launch(bg_kernel_t kernel, const bg_group_args_t *args, uint32_t grid_blocks,
+                    void *stream);
 
 /* LSB-first u64 words -> the reference's u32 layout (position p = bit
  * 31 - p%32 of word p/32, bitgen/bitstream.py:30-52).  dst holds 2*n_words
@@ -139,16 +174,16 @@ int bg_to_reference_layout(const uint64_t *src, uint32_t *dst, uint64_t n_words,
 int bg_popcount(const uint64_t *src, uint64_t n_words, uint64_t *count, void *stream);
 
 /* Batched launch (no counterpart in the reference, which launches one grid
- * of (groups x inputs) blocks per pattern set): enqueues a one-thread
- * prologue kernel that resets ctl[0] and advances the epoch ctl[2] on the
- * device -- so the pair can be captured in a graph and replayed -- then
- * grid_blocks blocks of block_threads threads of the persistent batch kernel. */
+ * of (groups x inputs) blocks per pattern set): grid_blocks blocks (0 = as
+ * many as fit at once) of block_threads threads of the persistent batch
+ * kernel.  A single kernel node, so it can be captured in a graph and
+ * replayed (the control block resets itself at the end of every launch). */
 int bg_batch_launch(bg_kernel_t kernel, const bg_batch_args_t *args, uint32_t grid_blocks,
                     uint32_t block_threads, void *stream);
 
 /* Stream capture into a replayable graph (hipStreamBeginCapture /
  * hipStreamEndCapture / hipGraphInstantiate / hipGraphLaunch).  Work captured
- * must not contain memsets (the batch prologue is a kernel for that reason). */
+ * must not contain memsets. */
 int bg_graph_begin(void *stream);
 int bg_graph_end(void *stream, bg_graph_t *out);
 int bg_graph_launch(bg_graph_t graph, void *stream);

@@ -62,6 +62,52 @@ def test_generated_source_shape():
     assert "__syncthreads" not in src  # waves never synchronise
 
 
+@pytest.mark.parametrize("regexes", [
+    ["a.{0,3}b", "[0-9]{2,4}", "a(bc){0,3}d", "(ab|cd){1,3}e"],
+    ["c?d"],
+    ["ab[^\\n]*cd", "x.*y", "q[0-9]+z"],
+    ["hello", "w.*d", "^a[0-9]*$"],
+])
+def test_pass_loop_replays_exactly(regexes):
+    """The speculative pass loop: every variable assigned inside the loop is
+    reset at its top and none is also assigned before it, so a replay pass
+    recomputes what one straight run computes (header values are read-only
+    inside the loop)."""
+    p = _prog(regexes)
+    src = p.source()
+    assert p.n_carry >= 1 and p.loop_start is not None
+    before = {d for i in p.insts[:p.loop_start] for d in i.defs()}
+    inside = {d for i in p.insts[p.loop_start:] for d in i.defs()}
+    assert not (before & inside)
+    loop = src[src.index("for (;;) {"):]
+    resets = loop[:loop.index("out = 0;")]
+    from bitgen_amd.codegen.gen_hip import _cname
+    for v in inside:
+        assert f"{_cname(v)} = 0;" in resets
+    assert src.count("goto bg_pass_end;") == p.n_carry
+    assert f"bg_settle<{p.n_carry}>(tc)" in src
+
+
+def test_carry_bounds_follow_data_flow():
+    """Speculation bounds: a carry op's operand bound only involves the carry
+    ops its operands are computed from."""
+    p = _prog(["ab[^\\n]*cd", "x.*y"])
+    p.source()
+    k_last = p.n_carry - 1
+    # the stars of independent regexes depend on no earlier carry op
+    for k in range(k_last):
+        assert p.carry_bounds[k] == {}
+    # the line projection depends on every star (through the match streams)
+    assert set(p.carry_bounds[k_last]) == set(range(k_last))
+    # chained stars: the second star's operand comes from the first
+    q = _prog(["a.*b.*c"])
+    q.source()
+    assert q.n_carry >= 3
+    assert 0 in q.carry_bounds[1]
+    assert "bg_infl_0" in q._bound_expr(1)
+    assert q._bound_expr(0) == "-1"
+
+
 @pytest.mark.skipif(not os.path.exists("/opt/rocm/bin/hipcc"), reason="hipcc not installed")
 def test_generated_kernel_compiles_for_gfx950(tmp_path):
     src = _prog(["ab[0-9]{1,3}c", "q.*z", "a.{80}b"]).source()
@@ -74,7 +120,7 @@ def test_generated_kernel_compiles_for_gfx950(tmp_path):
 
 def _declared_symbols():
     text = open(os.path.join(ROOT, "include", "bitgen_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(bg_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint32_t|const char \*)\s*(bg_\w+)\s*\(", text, re.M)))
 
 
 def test_abi_library_exports_every_declared_symbol():
@@ -92,20 +138,33 @@ def test_abi_library_exports_every_declared_symbol():
     rt.load_library().bg_last_error()
 
 
-def test_group_args_struct_matches_header(tmp_path):
-    """ctypes mirror vs the C header: sizes and offsets compiled with gcc."""
-    from bitgen_amd.backend.hip.runtime import GroupArgs
+def test_abi_structs_and_ctl_layout_match_header(tmp_path):
+    """ctypes mirrors vs the C header: sizes and offsets of every struct and
+    the control-block word indices, compiled with gcc."""
+    from bitgen_amd.backend.hip import runtime as rt
 
+    structs = {"bg_group_args_t": rt.GroupArgs, "bg_batch_args_t": rt.BatchArgs, "bg_job_t": rt.Job,
+               "bg_tdesc_t": rt.TDesc, "bg_kernel_info_t": rt.KernelInfo}
+    body = []
+    for cname, py in structs.items():
+        body.append(f'printf("%zu ", sizeof({cname}));')
+        body += [f'printf("%zu ", offsetof({cname}, {f}));' for f, _ in py._fields_]
+    consts = ["BG_MAX_QUEUES", "BG_CTL_STRIDE", "BG_CTL_QDONE", "BG_CTL_ERROR", "BG_CTL_EPOCH", "BG_CTL_WORDS",
+              "BG_CTL_COUNTER(3)", "BG_CTL_DONE(3)"]
+    body += [f'printf("%d ", (int)({c}));' for c in consts]
     src = tmp_path / "s.c"
-    fields = [f for f, _ in GroupArgs._fields_]
-    body = "".join(f'printf("%zu ", offsetof(bg_group_args_t, {f}));' for f in fields)
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "bitgen_hip.h"\n'
-                   f'int main(void) {{ printf("%zu ", sizeof(bg_group_args_t)); {body} return 0; }}\n')
+                   f'int main(void) {{ {"".join(body)} return 0; }}\n')
     exe = tmp_path / "s"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
-    assert got[0] == ctypes.sizeof(GroupArgs)
-    assert got[1:] == [getattr(GroupArgs, f).offset for f in fields]
+    want = []
+    for py in structs.values():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    want += [rt.MAX_QUEUES, rt.CTL_STRIDE, rt.CTL_QDONE, rt.CTL_ERROR, rt.CTL_EPOCH, rt.CTL_WORDS,
+             3 * rt.CTL_STRIDE, (rt.MAX_QUEUES + 3) * rt.CTL_STRIDE]
+    assert got == want
 
 
 def test_transpose_core_host_build(tmp_path):

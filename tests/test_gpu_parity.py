@@ -71,3 +71,32 @@ def test_long_shifts_use_multi_word_halo(gpu):
     data = rng.choice(np.frombuffer(b"abcx\n", dtype=np.uint8), size=200_000, p=[.2, .2, .2, .39, .01])
     groups = [["a.{0,100}b"], ["a.{150}b", "c[ab]{70,90}c"], ["ab.{0,40}ba.{0,40}ab.{0,40}c"]]
     assert_parity(groups, data, gpu, label="long shifts")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_waiting_lookback_matches_oracle(gpu, seed):
+    """The waiting look-back (every carry op waits for its predecessor), the
+    form used for programs with more than 64 carry ops, on star-heavy groups
+    with long and short lines."""
+    rng = random.Random(100 + seed)
+    alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
+    p = [0.9 / 12] * 12 + [0.1] if seed % 2 else [1 / 13] * 13
+    data = np.random.default_rng(seed).choice(alpha, size=rng.choice([9000, 300001]), p=p)
+    groups = [["a.*b", "x[^\\n]*y", "q(a|b)z*[xy]{2}"], ["ab[0-9]*c", "d.*3"]]
+    print(f"seed={seed} n={data.size}", flush=True)
+    assert_parity(groups, data, gpu, options={"lookback": "wait"}, label=f"wait seed {seed}")
+
+
+def test_more_than_64_carry_ops(gpu):
+    """A group whose program has more than 64 carry ops (stars of 70
+    regexes) runs with the waiting look-back."""
+    from bitgen_amd import compiler
+    from bitgen_amd.codegen import GroupProgram
+
+    regs = [f"{chr(97 + i % 26)}{i % 10}.*{chr(97 + (i * 7) % 26)}" for i in range(70)]
+    _, _, post, vnm = compiler.group_programs(regs)
+    assert GroupProgram(0, post, vnm, regs).n_carry > 64
+    rng = np.random.default_rng(5)
+    alpha = np.frombuffer(b"abcdefghij0123456789 \n", dtype=np.uint8)
+    data = rng.choice(alpha, size=150_000)
+    assert_parity([regs], data, gpu, label="70 stars")
