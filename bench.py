@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--groups", type=int, default=16)
     ap.add_argument("--queues", type=int, default=0, help="tile queues per launch (0 = runtime default)")
     ap.add_argument("--grid-blocks", type=int, default=0, help="persistent blocks per launch (0 = all that fit)")
+    ap.add_argument("--take", type=int, default=0, help="tiles per queue take (0 = runtime default)")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="compiler option override (e.g. pass_short_circuit_interval=2)")
     ap.add_argument("--cpu-sample-mib", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check the GPU result against the oracle on a slice")
@@ -93,6 +96,21 @@ def cpu_baseline(groups, buf, sample_bytes):
     }
 
 
+def profiled_traffic(args):
+    """HBM bytes per group-kernel launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, scaled by a plain copy kernel) for the default workload, from
+    the committed counter summary; None for other workloads."""
+    if (args.mib, args.regexes, args.groups) != (100, 1000, 16):
+        return None
+    path = os.path.join(ROOT, "profiles", "bg_group_ids_100MiB_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(float(line.split("=")[1]))
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -115,12 +133,17 @@ def main():
     print(f"[bench] workload ready: {len(regexes)} regexes, {len(groups)} groups, {n_bytes} bytes "
           f"({time.perf_counter() - t_setup:.1f}s)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    cs = compiler.compile_pattern_set(groups)
+    options = {}
+    for kv in args.option:
+        k, v = kv.split("=", 1)
+        options[k] = int(v) if v.lstrip("-").isdigit() else v
+    cs = compiler.compile_pattern_set(groups, options)
     print(f"[bench] compiled {len(groups)} group kernels ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
     dev = torch.device("cuda", 0)
     ks = cs.load(0)
     ks.n_queues = args.queues
     ks.grid_blocks = args.grid_blocks
+    ks.take = args.take
     from bitgen_amd.backend.hip import runtime as rt_
 
     occ = [rt_.kernel_occupancy(k, 256)[0] for k in ks.kernels]
@@ -214,7 +237,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": None,
+            "traffic": profiled_traffic(args),
         },
     }
     if count is not None:

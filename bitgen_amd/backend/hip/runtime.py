@@ -26,7 +26,7 @@ class GroupArgs(ctypes.Structure):
         ("n_tiles", ctypes.c_uint32),
         ("n_queues", ctypes.c_uint32),
         ("n_carry", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("take", ctypes.c_uint32),
     ]
 
 
@@ -55,7 +55,7 @@ class BatchArgs(ctypes.Structure):
         ("n_jobs", ctypes.c_uint32),
         ("total_tiles", ctypes.c_uint32),
         ("n_queues", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("take", ctypes.c_uint32),
     ]
 
 

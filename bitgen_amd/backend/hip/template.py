@@ -49,6 +49,7 @@ class HipGroupSet:
         # launches of a group are ordered on one stream, groups never share one
         self.ctl = torch.zeros((len(metas), rt.CTL_WORDS), dtype=torch.int32, device=dev)
         self.n_queues = 0  # 0 = the runtime's default
+        self.take = 0  # tiles per queue take, 0 = the runtime's default
         self.grid_blocks = 0  # 0 = as many blocks as fit on the device
 
     @property
@@ -85,6 +86,7 @@ class HipGroupSet:
         args.n_tiles = self.n_tiles(meta, n_words)
         args.n_queues = self.n_queues
         args.n_carry = max(1, meta["n_carry"])
+        args.take = self.take
         rt.group_launch(self.kernels[g], args, stream, self.grid_blocks)
 
     def run(self, basis, result, status, stream=None):

@@ -97,7 +97,7 @@ class BatchPlan:
     arrays; ``jobs``: list of (set index, buffer index) pairs.  Each pair
     expands to one job per group of the set."""
 
-    def __init__(self, sets, buffers, jobs, device=0, grid_blocks=None, n_queues=0):
+    def __init__(self, sets, buffers, jobs, device=0, grid_blocks=None, n_queues=0, take=0):
         import torch
 
         self.torch = torch
@@ -168,6 +168,7 @@ class BatchPlan:
             args.n_jobs = len(entries)
             args.total_tiles = q["total_tiles"]
             args.n_queues = n_queues
+            args.take = take
             q["args"] = args
         # one transpose launch for all buffers (a descriptor per buffer)
         td = (rt.TDesc * max(1, self.n_buffers))()

@@ -41,6 +41,41 @@ class HipCodegenError(NotImplementedError):
     pass
 
 
+_BITWISE = (BsInstType.AND, BsInstType.OR, BsInstType.XOR, BsInstType.NOT)
+# truth-table patterns of the three bitop3 inputs (v_bitop3_b32 convention:
+# result bit = LUT[(a << 2) | (b << 1) | c])
+_LUT_IN = (0xF0, 0xCC, 0xAA)
+
+
+def _lut(expr, leaves):
+    """Truth table (8 bits) of an expression tree over up to three leaves."""
+    kind = expr[0]
+    if kind == "leaf":
+        return _LUT_IN[leaves.index(expr[1])]
+    if kind == "const":
+        return 0xFF if expr[1] else 0x00
+    if kind == "not":
+        return ~_lut(expr[1], leaves) & 0xFF
+    a, b = _lut(expr[1], leaves), _lut(expr[2], leaves)
+    return {"and": a & b, "or": a | b, "xor": a ^ b}[kind]
+
+
+def _leaves(expr, out):
+    if expr[0] == "leaf":
+        if expr[1] not in out:
+            out.append(expr[1])
+    elif expr[0] != "const":
+        for e in expr[1:]:
+            _leaves(e, out)
+    return out
+
+
+def _nops(expr):
+    if expr[0] in ("leaf", "const"):
+        return 0
+    return 1 + sum(_nops(e) for e in expr[1:])
+
+
 def _cname(v):
     return f"v{v}"
 
@@ -205,13 +240,30 @@ class GroupProgram:
         body_vars = list(dict.fromkeys(d for i in self.insts[body_start:] for d in i.defs()))
         self.loop_start = body_start if speculative else None
         self.carry_bounds = self._carry_bounds(carry_index)
+        # operand bound of carry op k = max over the carry ops j it depends on
+        # of (j's result bound + lanes of right advance from j to k), kept as
+        # one running maximum per dependent op
+        consumers = {k: d for k, d in self.carry_bounds.items() if d}
+        feeds = {}
+        for k, d in consumers.items():
+            for j, bits in d.items():
+                feeds.setdefault(j, []).append((k, math.ceil(bits / WAVE)))
         if speculative:
-            add("  " + " ".join(f"int bg_infl_{k} = -1;" for k in range(self.n_carry)))
+            add("  int bg_infl = -1;" + "".join(f" int bg_bnd_{k} = -1;" for k in sorted(consumers)))
+        # three-input bitwise trees -> v_bitop3_b32
+        trees, absorbed = self._fuse()
+        self.fused = (trees, absorbed)
+
+        def eff_uses(k, inst):
+            if k in trees:
+                return _leaves(trees[k], [])
+            return inst.uses()
+
         # values shared by every regex (the basis and the group header defined
         # before the first zero test) are fenced at regex boundaries
         first_skip = next((k for k, i in enumerate(self.insts) if i.type == BsInstType.SKIPZERO), len(self.insts))
-        header = [d for i in self.insts[:first_skip] for d in i.defs()]
-        later = {u for i in self.insts[first_skip:] for u in i.uses()}
+        header = [d for k, i in enumerate(self.insts[:first_skip]) if k not in absorbed for d in i.defs()]
+        later = {u for k, i in enumerate(self.insts) if k >= first_skip and k not in absorbed for u in eff_uses(k, i)}
         fence_vars = [v for v in dict.fromkeys(header) if v in later]
         open_regions = set()
         for idx, inst in enumerate(self.insts):
@@ -220,18 +272,13 @@ class GroupProgram:
                 zeros = [f"{_cname(v)} = 0;" for v in body_vars]
                 for i in range(0, len(zeros), 16):
                     add("  " + " ".join(zeros[i:i + 16]))
-                add("  out = 0;")
+                add("  out = 0;" + "".join(f" bg_bnd_{k} = -1;" for k in sorted(consumers)))
             t = inst.type
             if t == BsInstType.STREAMLOAD:
                 add(f"  {_cname(inst.ret)} = bas[{int(inst.stream_index)}];")
-            elif t == BsInstType.AND:
-                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])} & {self.val(inst.operands[1])};")
-            elif t == BsInstType.OR:
-                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])} | {self.val(inst.operands[1])};")
-            elif t == BsInstType.XOR:
-                add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])} ^ {self.val(inst.operands[1])};")
-            elif t == BsInstType.NOT:
-                add(f"  {_cname(inst.ret)} = ~{self.val(inst.operands[0])};")
+            elif t in _BITWISE:
+                if idx not in absorbed:
+                    add(f"  {_cname(inst.ret)} = {self._emit_tree(trees[idx])};")
             elif t == BsInstType.ASSIGN:
                 add(f"  {_cname(inst.ret)} = {self.val(inst.operands[0])};")
             elif t == BsInstType.ADVANCE:
@@ -241,9 +288,12 @@ class GroupProgram:
                 k, L = carry_index[id(inst)]
                 fn = "bg_matchstar" if t == BsInstType.MATCHSTAR else "bg_scanthru"
                 if speculative:
+                    bnd = f"bg_bnd_{k}" if k in consumers else "-1"
                     add(f"  {_cname(inst.ret)} = {fn}<{L}, S, {k}>({self.val(inst.operands[0])}, "
-                        f"{self.val(inst.operands[1])}, tc, {self._bound_expr(k)}, bg_infl_{k});")
+                        f"{self.val(inst.operands[1])}, tc, {bnd}, bg_infl);")
                     add("  if (tc.abort) goto bg_pass_end;")
+                    for c, lanes in feeds.get(k, []):
+                        add(f"  bg_bnd_{c} = bg_imax(bg_bnd_{c}, bg_infl" + (f" + {lanes});" if lanes else ");"))
                 else:
                     add(f"  {{ int bg_unused = -1; {_cname(inst.ret)} = {fn}<{L}, S, {k}>("
                         f"{self.val(inst.operands[0])}, {self.val(inst.operands[1])}, tc, -1, bg_unused); }}")
@@ -272,6 +322,125 @@ class GroupProgram:
         add("  if (inb && lane >= lo && lane < hi) result[w] = out;")
         add("}")
         return "\n".join(lines) + "\n"
+
+    def _fuse(self):
+        """Bitwise expression trees for v_bitop3_b32 (three-input lookup
+        table): an AND/OR/XOR/NOT result that feeds exactly one other bitwise
+        op, and any NOT, is folded into its consumer while the consumer's tree
+        has at most three distinct inputs.  Only single-definition values are
+        folded, and a folded value's own inputs must not be reassigned between
+        its definition and the consumer, so moving the computation to the
+        consumer computes the same value.  Returns ({index: tree} for emitted
+        bitwise instructions, set of absorbed instruction indices)."""
+        insts = self.insts
+        ndefs = {}
+        def_at = {}
+        uses = {}
+        for k, inst in enumerate(insts):
+            for d in inst.defs():
+                ndefs[d] = ndefs.get(d, 0) + 1
+                def_at[d] = k
+            for u in inst.uses():
+                uses.setdefault(u, []).append(k)
+        defs_of = {}
+        for k, inst in enumerate(insts):
+            for d in inst.defs():
+                defs_of.setdefault(d, []).append(k)
+        # skip regions open at each instruction: a value defined inside a
+        # region is only folded into consumers inside that region
+        regions, open_ = [], set()
+        for inst in insts:
+            if inst.type == BsInstType.LABEL:
+                open_.discard(inst.label)
+            regions.append(frozenset(open_))
+            if inst.type == BsInstType.SKIPZERO:
+                open_.add(inst.label)
+
+        def stable(v, lo, hi):
+            # v keeps its value from instruction lo to instruction hi
+            return all(not (lo < k < hi) for k in defs_of.get(v, ()))
+
+        def foldable(v, consumer):
+            if self._const(v) is not None or ndefs.get(v) != 1:
+                return False
+            k = def_at[v]
+            inst = insts[k]
+            if k >= consumer or inst.type not in _BITWISE or not regions[k] <= regions[consumer]:
+                return False
+            if inst.type != BsInstType.NOT and len(uses.get(v, ())) != 1:
+                return False
+            return all(self._const(o) is not None or stable(o, k, consumer) for o in inst.operands)
+
+        absorbed_uses = {}
+        trees = {}
+        absorbed = set()
+
+        def leaf(v):
+            c = self._const(v)
+            return ("const", c == "1") if c is not None else ("leaf", v)
+
+        def expand(v, consumer, budget):
+            """Tree for operand v of `consumer`, folding while the consumer's
+            distinct inputs stay within three."""
+            if not foldable(v, consumer):
+                return leaf(v), []
+            inst = insts[def_at[v]]
+            kind = {BsInstType.AND: "and", BsInstType.OR: "or", BsInstType.XOR: "xor", BsInstType.NOT: "not"}[inst.type]
+            subs, folded = [], [v]
+            inner = list(budget)
+            ops = list(inst.operands)
+            for i, o in enumerate(ops):
+                reserve = list(inner)
+                for later_op in ops[i + 1:]:
+                    reserve = _leaves(leaf(later_op), reserve)
+                e, f = expand(o, consumer, reserve)
+                subs.append(e)
+                folded += f
+                inner = _leaves(e, inner)
+            tree = (kind, *subs)
+            if len(_leaves(tree, list(budget))) > 3:
+                return leaf(v), []
+            return tree, folded
+
+        for k in range(len(insts) - 1, -1, -1):
+            inst = insts[k]
+            if inst.type not in _BITWISE or k in absorbed:
+                continue
+            kind = {BsInstType.AND: "and", BsInstType.OR: "or", BsInstType.XOR: "xor", BsInstType.NOT: "not"}[inst.type]
+            subs, folded_all, budget = [], [], []
+            ops = list(inst.operands)
+            for i, o in enumerate(ops):
+                # keep room for the operands still to come (each at least a leaf)
+                reserve = list(budget)
+                for later_op in ops[i + 1:]:
+                    reserve = _leaves(leaf(later_op), reserve)
+                e, f = expand(o, k, reserve)
+                subs.append(e)
+                folded_all += f
+                budget = _leaves(e, budget)
+            tree = (kind, *subs)
+            assert len(_leaves(tree, [])) <= 3, tree
+            trees[k] = tree
+            for v in folded_all:
+                absorbed_uses[v] = absorbed_uses.get(v, 0) + 1
+                if absorbed_uses[v] == len(uses.get(v, ())):
+                    absorbed.add(def_at[v])
+        return trees, absorbed
+
+    def _emit_tree(self, tree):
+        """C expression of a fused tree: plain C for a single two-input op,
+        else one bg_op3 lookup (two v_bitop3_b32)."""
+        kind = tree[0]
+        if _nops(tree) <= 1 and all(e[0] in ("leaf", "const") for e in tree[1:]):
+            vals = [("~0ull" if e[1] else "0ull") if e[0] == "const" else _cname(e[1]) for e in tree[1:]]
+            if kind == "not":
+                return f"~{vals[0]}"
+            return f"{vals[0]} {dict(andop='&', orop='|', xorop='^')[kind + 'op']} {vals[1]}"
+        leaves = _leaves(tree, [])
+        if not leaves:
+            return "~0ull" if _lut(tree, []) & 1 else "0ull"
+        args = [_cname(v) for v in leaves] + [_cname(leaves[0])] * (3 - len(leaves))
+        return f"bg_op3<0x{_lut(tree, leaves):02X}>({', '.join(args)})"
 
     def _carry_bounds(self, carry_index):
         """For each carry op k: {earlier carry op j: lanes of right advance
@@ -309,19 +478,6 @@ class GroupProgram:
             merge(cur, res)
         return bounds
 
-    def _bound_expr(self, k):
-        """C expression of carry op k's operand bound (see bg_wave_add)."""
-        terms = []
-        for j, bits in sorted(self.carry_bounds.get(k, {}).items()):
-            lanes = math.ceil(bits / WAVE)
-            terms.append(f"bg_infl_{j}" + (f" + {lanes}" if lanes else ""))
-        if not terms:
-            return "-1"
-        e = terms[0]
-        for t in terms[1:]:
-            e = f"bg_imax({e}, {t})"
-        return e
-
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
         return "\n".join([
@@ -330,14 +486,15 @@ class GroupProgram:
             f'extern "C" __global__ __launch_bounds__(256) void {self.kernel_name}(bg_group_args_t a) {{',
             "  // persistent: each wave takes tiles from its block's queue until the",
             "  // queue is exhausted",
-            "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles);",
-            "  uint32_t t;",
-            "  bool more = bg_queue_take(Q, t);",
+            "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles, a.take);",
+            "  uint32_t first;",
+            "  bool more = bg_queue_take(Q, first);",
             "  while (more) {",
-            "    const uint32_t next = bg_queue_issue(Q);  // next tile, taken ahead",
-            f"    bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
+            "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
+            "    for (uint32_t i = 0, t = first; i < Q.take && t < Q.n; i++, t += Q.nq)",
+            f"      bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
             "a.ctl + BG_CTL_ERROR);",
-            "    more = bg_queue_finish(Q, next, t);",
+            "    more = bg_queue_finish(Q, next, first);",
             "  }",
             "  bg_queue_close(Q);",
             "}",
@@ -371,16 +528,18 @@ def batch_source(program, block_threads=256):
         '#include "bitgen_device.h"',
         program.device_function(),
         f'extern "C" __global__ __launch_bounds__({block_threads}) void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
-        "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles);",
-        "  uint32_t tg;",
-        "  bool more = bg_queue_take(Q, tg);",
+        "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles, a.take);",
+        "  uint32_t first;",
+        "  bool more = bg_queue_take(Q, first);",
         "  while (more) {",
-        "    const uint32_t next = bg_queue_issue(Q);  // next tile, taken ahead",
-        "    const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
-        "    const bg_job_t J = a.jobs[j];",
-        f"    bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, Q.epoch, J.n_carry, "
+        "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
+        "    for (uint32_t i = 0, tg = first; i < Q.take && tg < Q.n; i++, tg += Q.nq) {",
+        "      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
+        "      const bg_job_t J = a.jobs[j];",
+        f"      bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, Q.epoch, J.n_carry, "
         "a.ctl + BG_CTL_ERROR);",
-        "    more = bg_queue_finish(Q, next, tg);",
+        "    }",
+        "    more = bg_queue_finish(Q, next, first);",
         "  }",
         "  bg_queue_close(Q);",
         "}",

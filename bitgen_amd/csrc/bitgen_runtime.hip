@@ -257,6 +257,11 @@ int bg_kernel_info(bg_kernel_t k, bg_kernel_info_t *info) {
 
 namespace {
 
+uint32_t clamp_take(uint32_t take) {
+  if (take == 0) take = 1;
+  return take > 8 ? 8 : take;
+}
+
 // Blocks of `threads` threads that fit on one CU (cached per kernel).
 int occupancy(bg_kernel_t k, uint32_t threads, int32_t *blocks_per_cu) {
   if (k->occ_threads != threads) {
@@ -272,8 +277,9 @@ int occupancy(bg_kernel_t k, uint32_t threads, int32_t *blocks_per_cu) {
 
 // Persistent grid: the blocks that fit on the device at once (or the
 // caller's count), never more than one wave per tile.
-int persistent_grid(bg_kernel_t k, uint32_t threads, uint32_t n_tiles, uint32_t requested, uint32_t *blocks) {
-  const uint32_t wpb = threads / 64;
+int persistent_grid(bg_kernel_t k, uint32_t threads, uint32_t n_tiles, uint32_t take, uint32_t requested,
+                    uint32_t *blocks) {
+  const uint32_t wpb = threads / 64 * take;  // tiles per block and round
   uint32_t b = requested;
   if (b == 0) {
     int32_t per_cu = 1;
@@ -286,6 +292,7 @@ int persistent_grid(bg_kernel_t k, uint32_t threads, uint32_t n_tiles, uint32_t 
   *blocks = b ? b : 1;
   return 0;
 }
+
 
 uint32_t clamp_queues(uint32_t nq, uint32_t blocks) {
   if (nq == 0) nq = 8;
@@ -315,10 +322,11 @@ int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t grid_bl
   if (args->n_tiles == 0) return 0;
   const uint32_t threads = 256;
   uint32_t blocks = 0;
-  int rc = persistent_grid(k, threads, args->n_tiles, grid_blocks, &blocks);
+  int rc = persistent_grid(k, threads, args->n_tiles, clamp_take(args->take), grid_blocks, &blocks);
   if (rc) return rc;
   bg_group_args_t a = *args;
   a.n_queues = clamp_queues(a.n_queues, blocks);
+  a.take = clamp_take(a.take);
   void *params[] = {&a};
   BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, threads, 1, 1, 0, S(stream), params, nullptr),
          "hipModuleLaunchKernel");
@@ -356,10 +364,11 @@ int bg_batch_launch(bg_kernel_t k, const bg_batch_args_t *args, uint32_t grid_bl
     return set_err(BG_EINVAL, "bg_batch_launch: block_threads must be a multiple of 64, at most 1024");
   if (args->total_tiles == 0) return 0;
   uint32_t blocks = 0;
-  int rc = persistent_grid(k, block_threads, args->total_tiles, grid_blocks, &blocks);
+  int rc = persistent_grid(k, block_threads, args->total_tiles, clamp_take(args->take), grid_blocks, &blocks);
   if (rc) return rc;
   bg_batch_args_t a = *args;
   a.n_queues = clamp_queues(a.n_queues, blocks);
+  a.take = clamp_take(a.take);
   void *params[] = {&a};
   BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, block_threads, 1, 1, 0, S(stream), params, nullptr),
          "hipModuleLaunchKernel(batch)");
@@ -378,7 +387,7 @@ int bg_graph_end(void *stream, bg_graph_t *out) {
   hipGraphExec_t e;
   hipError_t err = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
   if (err != hipSuccess) {
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);  // the instantiation error is the one reported
     return hip_err(err, "hipGraphInstantiate");
   }
   *out = new bg_graph_s{g, e};
@@ -393,9 +402,11 @@ int bg_graph_launch(bg_graph_t graph, void *stream) {
 
 int bg_graph_destroy(bg_graph_t graph) {
   if (!graph) return 0;
-  hipGraphExecDestroy(graph->exec);
-  hipGraphDestroy(graph->graph);
+  const hipError_t e1 = hipGraphExecDestroy(graph->exec);
+  const hipError_t e2 = hipGraphDestroy(graph->graph);
   delete graph;
+  BG_HIP(e1, "hipGraphExecDestroy");
+  BG_HIP(e2, "hipGraphDestroy");
   return 0;
 }
 

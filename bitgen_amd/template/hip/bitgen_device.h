@@ -125,6 +125,17 @@ __device__ __forceinline__ bool bg_wave_zero(uint64_t x) {
   return __builtin_amdgcn_ballot_w64(x != 0ull) == 0ull;
 }
 
+// ---- three-input bitwise functions ---------------------------------------------
+// f(a, b, c) for the 8-bit truth table LUT (bit (a << 2 | b << 1 | c) holds
+// f's value; a = 0xF0, b = 0xCC, c = 0xAA give the table), one v_bitop3_b32
+// per 32-bit half.
+template <unsigned LUT>
+__device__ __forceinline__ uint64_t bg_op3(uint64_t a, uint64_t b, uint64_t c) {
+  const uint32_t lo = __builtin_amdgcn_bitop3_b32((uint32_t)a, (uint32_t)b, (uint32_t)c, LUT);
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32((uint32_t)(a >> 32), (uint32_t)(b >> 32), (uint32_t)(c >> 32), LUT);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // ---- tile context ----------------------------------------------------------
 // Geometry (words):  tile 0 covers words [0, 64) and outputs lanes
 // [0, 64 - HR);  tile t >= 1 covers [S0 + (t-1)*S - HL, +64) and outputs lanes
@@ -339,21 +350,22 @@ __device__ __forceinline__ bool bg_settle(bg_tile &tc) {
   static_assert(NC >= 1 && NC <= 64, "programs with more than 64 carry operations start in replay mode");
   if (tc.t == 0 || tc.mode == BG_REPLAY) return true;
   const uint32_t n = tc.abort ? tc.k_exact : (uint32_t)NC;
+  // one load instruction: lane k reads the predecessor's status of op k
+  const int lane = bg_lane();
   const uint32_t *prev = tc.status + (size_t)(tc.t - 1) * tc.n_carry;
-  uint32_t st[NC];
-#pragma unroll
-  for (int k = 0; k < NC; k++) st[k] = ((uint32_t)k < n) ? bg_st_load(prev + k) : 0u;
-  uint64_t cin = 0;
-#pragma unroll
-  for (int k = 0; k < NC; k++) {
-    if ((uint32_t)k >= n) break;
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)st[k]);
-    uint32_t c;
-    if ((w >> 4) == (tc.epoch & 0x0FFFFFFFu) && ((w >> 2) & 3u) == BG_ST_INC)
-      c = w & 1u;
-    else
-      c = bg_lookback(tc, (uint32_t)k, w);
-    cin |= (uint64_t)c << k;
+  uint32_t w = 0;
+  if ((uint32_t)lane < n) w = bg_st_load(prev + lane);
+  const bool inc = (uint32_t)lane < n && (w >> 4) == (tc.epoch & 0x0FFFFFFFu) && ((w >> 2) & 3u) == BG_ST_INC;
+  const uint64_t inc_ops = __builtin_amdgcn_ballot_w64(inc);
+  uint64_t cin = __builtin_amdgcn_ballot_w64(inc && (w & 1u));
+  // predecessors that had not published yet, or published an aggregate:
+  // the waiting look-back, op by op (rare)
+  uint64_t rest = ((n >= 64) ? ~0ull : ((1ull << n) - 1ull)) & ~inc_ops;
+  while (rest) {
+    const int k = __builtin_ctzll(rest);
+    rest &= rest - 1;
+    const uint32_t wk = (uint32_t)__builtin_amdgcn_readlane((int)w, k);
+    cin |= (uint64_t)bg_lookback(tc, (uint32_t)k, wk) << k;
   }
   if (tc.agg) {
     uint64_t a = tc.agg;
@@ -402,6 +414,7 @@ struct bg_queue {
   uint32_t *ctl;
   uint32_t q;       // this block's queue
   uint32_t nq;      // queues of the launch
+  uint32_t take;    // tiles per take (consecutive tiles of the queue)
   uint32_t n;       // tiles of the launch
   uint32_t epoch;   // status tag of the launch
 };
@@ -419,7 +432,7 @@ struct bg_queue_lds {
 // Device-scope accesses to one shared word serialise, so each block makes
 // one of each: a single epoch read at the start (shared through LDS) and a
 // single done-increment when its last wave leaves.
-__device__ __forceinline__ bg_queue bg_queue_open(uint32_t *ctl, uint32_t nq, uint32_t n_tiles) {
+__device__ __forceinline__ bg_queue bg_queue_open(uint32_t *ctl, uint32_t nq, uint32_t n_tiles, uint32_t take) {
   __shared__ bg_queue_lds lds;
   bg_queue Q;
   Q.lds = &lds;
@@ -428,6 +441,7 @@ __device__ __forceinline__ bg_queue bg_queue_open(uint32_t *ctl, uint32_t nq, ui
   // queue is served by at least one block whatever the caller passed
   Q.nq = (nq == 0) ? 1u : (nq < gridDim.x ? nq : gridDim.x);
   Q.q = blockIdx.x % Q.nq;
+  Q.take = (take == 0) ? 1u : take;
   Q.n = n_tiles;
   if (threadIdx.x == 0) {
     lds.epoch = bg_ctl_load(ctl + BG_CTL_EPOCH) & 0x0FFFFFFFu;
@@ -438,14 +452,15 @@ __device__ __forceinline__ bg_queue bg_queue_open(uint32_t *ctl, uint32_t nq, ui
   return Q;
 }
 
-// Taking a tile is split in two so that a wave can ask for its next tile
-// before running the current one (the atomic's latency then overlaps the
-// tile's own loads): bg_queue_issue sends the atomic add (the old value
-// lands in lane 0), bg_queue_finish turns it into a tile index and returns
+// One take gives a wave `take` consecutive tiles of its queue: tiles
+// g, g + nq, ..., g + (take-1)*nq with g = k*take*nq + q for the counter
+// value k.  Taking is split in two so that a wave can ask for its next tiles
+// before running the current ones (the atomic's latency then overlaps the
+// tiles' own loads): bg_queue_issue sends the atomic add (the old value lands
+// in lane 0), bg_queue_finish turns it into the first tile g and returns
 // false when the queue is exhausted.  Each wave makes exactly one failing
-// take.  A wave holds at most its current tile and the next, larger one, so
-// the smallest unfinished tile is always being run and every look-back wait
-// ends.
+// take.  A wave runs the tiles it holds in increasing order, so the smallest
+// unfinished tile is always being run and every look-back wait ends.
 __device__ __forceinline__ uint32_t bg_queue_issue(const bg_queue &Q) {
   uint32_t k = 0;
   if (bg_lane() == 0)
@@ -455,7 +470,7 @@ __device__ __forceinline__ uint32_t bg_queue_issue(const bg_queue &Q) {
 
 __device__ __forceinline__ bool bg_queue_finish(const bg_queue &Q, uint32_t k, uint32_t &t) {
   k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
-  const uint64_t tt = (uint64_t)k * Q.nq + Q.q;
+  const uint64_t tt = (uint64_t)k * Q.take * Q.nq + Q.q;
   t = (uint32_t)tt;
   return tt < Q.n;
 }

@@ -26,12 +26,12 @@ typedef struct bg_kernel_s *bg_kernel_t;
 
 /* Tile queues.  A launch's tiles are dealt round-robin over n_queues
  * queues (tile t belongs to queue t % n_queues); block b takes tiles from
- * queue b % n_queues, each wave one tile at a time with one atomic add on its
- * queue's counter.  Within a queue tiles are taken in increasing order, and
- * a wave only ever waits (carry look-back) for smaller tiles, so no wave waits
- * on a tile that nobody can take once the first n_queues blocks are
- * resident.  One counter per queue spreads the atomics over n_queues
- * addresses (a single device-wide counter serialises them).
+ * queue b % n_queues, each wave `take` consecutive tiles of its queue per
+ * atomic add on the queue's counter.  Within a queue tiles are taken in
+ * increasing order, and a wave only ever waits (carry look-back) for smaller
+ * tiles, so no wave waits on a tile that nobody can take once the first
+ * n_queues blocks are resident.  One counter per queue spreads the atomics
+ * over n_queues addresses (a single device-wide counter serialises them).
  *
  * Control block ("ctl"): BG_CTL_WORDS u32 words, zeroed once before first
  * use and then owned by one launch at a time (launches that share a ctl must
@@ -62,7 +62,7 @@ typedef struct bg_group_args {
   uint32_t n_tiles;      /* tiles covering n_words                           */
   uint32_t n_queues;     /* 0 = default; clamped to 1 .. min(32, blocks)     */
   uint32_t n_carry;      /* carry operations in the program (>= 1 slot)      */
-  uint32_t reserved;
+  uint32_t take;         /* tiles per atomic take, 0 = default; 1 .. 8       */
 } bg_group_args_t;
 
 /* One (group, buffer) job of a batched launch.  Field order is ABI. */
@@ -88,7 +88,7 @@ typedef struct bg_batch_args {
   uint32_t n_jobs;
   uint32_t total_tiles;
   uint32_t n_queues;        /* 0 = default; clamped like bg_group_args     */
-  uint32_t reserved;
+  uint32_t take;            /* tiles per atomic take, as in bg_group_args  */
 } bg_batch_args_t;
 
 typedef struct bg_graph_s *bg_graph_t;

@@ -101,11 +101,15 @@ def test_carry_bounds_follow_data_flow():
     assert set(p.carry_bounds[k_last]) == set(range(k_last))
     # chained stars: the second star's operand comes from the first
     q = _prog(["a.*b.*c"])
-    q.source()
+    src = q.source()
     assert q.n_carry >= 3
     assert 0 in q.carry_bounds[1]
-    assert "bg_infl_0" in q._bound_expr(1)
-    assert q._bound_expr(0) == "-1"
+    # op 0 has no operand bound; op 1's bound is raised right after op 0
+    assert "bg_matchstar<0, S, 0>(" in src or "bg_scanthru<0, S, 0>(" in src
+    first = src[src.index(", S, 0>("):]
+    assert ", tc, -1, bg_infl);" in first.split("\n")[0]
+    assert "bg_bnd_1 = bg_imax(bg_bnd_1, bg_infl" in src
+    assert ", tc, bg_bnd_1, bg_infl);" in src
 
 
 @pytest.mark.skipif(not os.path.exists("/opt/rocm/bin/hipcc"), reason="hipcc not installed")
@@ -192,3 +196,116 @@ int main() {
     subprocess.run(["g++", "-O2", "-I", os.path.join(ROOT, "bitgen_amd", "csrc"), str(src), "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.strip() == "0"
+
+
+def _interp(prog, data_bits, n, fused):
+    """Whole-stream interpreter of a generated program (Python ints as
+    n-bit streams, bit p = position p).  With fused=True, bitwise
+    instructions are evaluated through the generator's bitop3 trees and
+    absorbed instructions are skipped, as in the emitted code."""
+    from bitgen_amd.inst import BsInstType as T
+    from bitgen_amd.codegen.gen_hip import _LUT_IN, _lut, _leaves
+
+    mask = (1 << n) - 1
+    trees, absorbed = prog._fuse()
+    env = {}
+
+    def val(v):
+        c = prog._const(v)
+        if c is not None:
+            return mask if c == "1" else 0
+        return env.get(v, 0)
+
+    def lut_eval(tree):
+        leaves = _leaves(tree, [])
+        lut = _lut(tree, leaves) if leaves else (0xFF if _lut(tree, []) & 1 else 0)
+        vals = [val(v) for v in leaves] + [val(leaves[0]) if leaves else 0] * (3 - len(leaves))
+        out = 0
+        for idx in range(8):
+            if (lut >> idx) & 1:
+                a, b, c = (idx >> 2) & 1, (idx >> 1) & 1, idx & 1
+                term = (vals[0] if a else ~vals[0]) & (vals[1] if b else ~vals[1]) & (vals[2] if c else ~vals[2])
+                out |= term
+        return out & mask
+
+    skip_to = None
+    out = None
+    for k, inst in enumerate(prog.insts):
+        t = inst.type
+        if skip_to is not None:
+            if t == T.LABEL and inst.label == skip_to:
+                skip_to = None
+            continue
+        if t == T.STREAMLOAD:
+            env[inst.ret] = data_bits[int(inst.stream_index)]
+        elif t in (T.AND, T.OR, T.XOR, T.NOT):
+            if fused:
+                if k not in absorbed:
+                    env[inst.ret] = lut_eval(trees[k])
+            else:
+                a = val(inst.operands[0])
+                if t == T.NOT:
+                    env[inst.ret] = ~a & mask
+                else:
+                    b = val(inst.operands[1])
+                    env[inst.ret] = {T.AND: a & b, T.OR: a | b, T.XOR: a ^ b}[t]
+        elif t == T.ASSIGN:
+            env[inst.ret] = val(inst.operands[0])
+        elif t == T.ADVANCE:
+            a = val(inst.operands[0])
+            env[inst.ret] = ((a >> inst.amount) if inst.is_left else (a << inst.amount)) & mask
+        elif t == T.MATCHSTAR:
+            m, c = val(inst.operands[0]), val(inst.operands[1])
+            env[inst.ret] = ((((m & c) + c) ^ c) | m) & mask
+        elif t == T.SCANTHRU:
+            m, c = val(inst.operands[0]), val(inst.operands[1])
+            env[inst.ret] = ((m + c) & ~c) & mask
+        elif t == T.SKIPZERO:
+            if val(inst.operands[0]) == 0:
+                skip_to = inst.label
+        elif t == T.STREAMSTORE:
+            out = val(inst.operands[0])
+    return out
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_bitop3_fusion_computes_the_same_streams(seed):
+    """The generator's three-input trees (v_bitop3_b32) and the skipped
+    (absorbed) instructions give the same match stream as the instructions
+    evaluated one by one, on random programs and inputs."""
+    import random
+    import numpy as np
+
+    from oracle import oracle as O
+    from tests.test_gpu_parity import POOL
+
+    rng = random.Random(seed)
+    regs = rng.sample(POOL, rng.randint(1, 6))
+    p = _prog(regs)
+    p.source()
+    from bitgen_amd.codegen.gen_hip import _leaves
+
+    trees, absorbed = p.fused
+    assert all(len(_leaves(t, [])) <= 3 for t in trees.values())
+    alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
+    data = np.random.default_rng(seed).choice(alpha, size=3000)
+    W = O.n_words_for(data.size)
+    basis = O.transpose(data, W)
+    n = 64 * W
+    bits = [int.from_bytes(basis[i].astype("<u8").tobytes(), "little") for i in range(8)]
+    assert _interp(p, bits, n, fused=True) == _interp(p, bits, n, fused=False)
+
+
+def test_bitop3_lut_convention():
+    """LUT bit (a << 2 | b << 1 | c) holds f(a, b, c): a few known tables."""
+    from bitgen_amd.codegen.gen_hip import _lut
+
+    a, b, c = ("leaf", "a"), ("leaf", "b"), ("leaf", "c")
+    L = ["a", "b", "c"]
+    assert _lut(("and", ("and", a, b), c), L) == 0x80
+    assert _lut(("or", ("or", a, b), c), L) == 0xFE
+    assert _lut(("xor", ("xor", a, b), c), L) == 0x96
+    # ~a & b: rows with a=0, b=1 -> indices 2 and 3
+    assert _lut(("and", ("not", a), b), L) == 0x0C
+    # a & ~b: rows with a=1, b=0 -> indices 4 and 5
+    assert _lut(("and", a, ("not", b)), L) == 0x30
