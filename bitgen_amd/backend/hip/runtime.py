@@ -60,7 +60,7 @@ class BatchArgs(ctypes.Structure):
 
 
 # Control block layout (BG_CTL_* in include/bitgen_hip.h): u32 word indices.
-MAX_QUEUES = 32
+MAX_QUEUES = 256
 CTL_STRIDE = 64
 CTL_QDONE = 2 * MAX_QUEUES * CTL_STRIDE
 CTL_ERROR = CTL_QDONE + CTL_STRIDE

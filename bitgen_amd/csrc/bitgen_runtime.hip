@@ -295,7 +295,7 @@ int persistent_grid(bg_kernel_t k, uint32_t threads, uint32_t n_tiles, uint32_t 
 
 
 uint32_t clamp_queues(uint32_t nq, uint32_t blocks) {
-  if (nq == 0) nq = 8;
+  if (nq == 0) nq = 64;
   if (nq > BG_MAX_QUEUES) nq = BG_MAX_QUEUES;
   if (nq > blocks) nq = blocks;
   return nq ? nq : 1;

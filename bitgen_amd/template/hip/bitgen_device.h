@@ -483,18 +483,29 @@ __device__ __forceinline__ bool bg_queue_take(const bg_queue &Q, uint32_t &t) {
 // reports the block; the last block of a queue resets the queue's counters;
 // the last queue to finish advances the epoch.  Every block read the epoch
 // before it reported, so the advance is seen only by the next launch.
+//
+// Every access is a relaxed device-scope atomic, with no fence: an
+// acquire/release at device scope makes the compiler write back and
+// invalidate the whole L2 of the block's XCD (buffer_wbl2 / buffer_inv),
+// which, done by every block at its end, cost more than all the tiles'
+// work.  The order the protocol needs comes from the waits instead: a wave
+// consumes the result of its failing take before it reports, the reporting
+// atomic returns before the last block resets, and device-scope atomics are
+// performed at the device's coherence point in that order.  The control
+// words carry no other data, and the next launch on the stream starts after
+// this one's end-of-kernel release.
 __device__ __forceinline__ void bg_queue_close(const bg_queue &Q) {
   if (bg_lane() != 0) return;
   const uint32_t wpb = blockDim.x / BG_WAVE;
   const uint32_t e = __hip_atomic_fetch_add(&Q.lds->exits, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   if (e + 1 != wpb) return;
   const uint32_t blocks_q = (gridDim.x - Q.q + Q.nq - 1) / Q.nq;
-  const uint32_t d = __hip_atomic_fetch_add(Q.ctl + BG_CTL_DONE(Q.q), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t d = __hip_atomic_fetch_add(Q.ctl + BG_CTL_DONE(Q.q), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (d + 1 != blocks_q) return;
   __hip_atomic_store(Q.ctl + BG_CTL_COUNTER(Q.q), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(Q.ctl + BG_CTL_DONE(Q.q), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t qd = __hip_atomic_fetch_add(Q.ctl + BG_CTL_QDONE, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t qd = __hip_atomic_fetch_add(Q.ctl + BG_CTL_QDONE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (qd + 1 != Q.nq) return;
   __hip_atomic_store(Q.ctl + BG_CTL_QDONE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(Q.ctl + BG_CTL_EPOCH, (Q.epoch + 1u) & 0x0FFFFFFFu, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(Q.ctl + BG_CTL_EPOCH, (Q.epoch + 1u) & 0x0FFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

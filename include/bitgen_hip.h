@@ -39,7 +39,7 @@ typedef struct bg_kernel_s *bg_kernel_t;
  * The last wave of each queue resets that queue's counters; the last queue to
  * finish advances the epoch, so consecutive launches (and graph replays)
  * need no host-side reset. */
-#define BG_MAX_QUEUES 32
+#define BG_MAX_QUEUES 256
 #define BG_CTL_STRIDE 64                                   /* u32 words   */
 #define BG_CTL_COUNTER(q) ((q) * BG_CTL_STRIDE)            /* tile counter */
 #define BG_CTL_DONE(q) ((BG_MAX_QUEUES + (q)) * BG_CTL_STRIDE) /* waves out */
