@@ -14,6 +14,21 @@ basis read (N bytes) + match stream write (ceil((N+1)/64)*8 bytes), divided
 by the average group-kernel duration measured with HIP events on the launch
 stream.  ``cpu_baseline`` times the oracle (the C executor of the same
 pre-pass IR) on a bounded sample of the same workload on this host.
+
+Two more measurements ride on the same line (the metric names them):
+
+* ``batched``: the batched-uneven workload -- pattern sets of 64, 256 and
+  1,024 regexes (groups of about 64) x 4,096 buffers of log-normal length
+  (median 16 KiB, clipped to 1 KiB-8 MiB), every (set, buffer) pair a job --
+  run as one graph-captured batch (one transpose launch plus one persistent
+  launch per group, all buffers) and as plain per-job launches (one
+  transpose plus one launch per group for every job); scanned GB/s of each
+  and their ratio.
+* ``served``: Poisson arrivals of jobs naming one of 8 resident pattern sets
+  (32 to 2,048 regexes) with log-normal buffers (median 4 KiB, max 64 MiB),
+  through the continuous-batching server and through synchronous per-job
+  plain launches on the same arrival schedule; p50/p99 latency (arrival to
+  result on the host) and GB/s of each.
 """
 import argparse
 import json
@@ -43,6 +58,12 @@ def parse():
     ap.add_argument("--cpu-sample-mib", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check the GPU result against the oracle on a slice")
+    ap.add_argument("--batched-buffers", type=int, default=4096)
+    ap.add_argument("--batched-mib", type=int, default=512, help="total bytes of the batched workload's buffers")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=3000, help="jobs per arrival rate")
+    ap.add_argument("--served-rates", default="5000,20000", help="Poisson arrival rates (jobs/s), comma separated")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (three times) and exit: the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -94,6 +115,195 @@ def cpu_baseline(groups, buf, sample_bytes):
         "sample": f"{sample.size / 2**20:.0f} MiB prefix of the same buffer, all {len(groups)} groups, "
                   f"oracle C executor, one thread per group ({dt:.2f} s wall)",
     }
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batched_measurement(args):
+    """Batched-uneven workload: one graph-captured batch vs plain per-job
+    launches over the same resident buffers (see the module docstring)."""
+    import numpy as np
+    import torch
+
+    from bitgen_amd import compiler, workloads
+    from bitgen_amd.backend.hip import runtime as rt
+    from bitgen_amd.batch import BatchPatternSet, BatchPlan, words_for
+
+    t0 = time.perf_counter()
+    sets, buffers, jobs = workloads.batched(n_buffers=args.batched_buffers, total_bytes=args.batched_mib * 2**20)
+    set_groups = [compiler.group_regex_in_list(s, workloads.groups_for(len(s))) for s in sets]
+    _log(f"batched workload ready: {len(buffers)} buffers, {sum(b.size for b in buffers)} bytes, sets "
+         f"{[len(s) for s in sets]} regexes in {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    bsets = [BatchPatternSet(g).load(0) for g in set_groups]
+    plan = BatchPlan(bsets, buffers, jobs)
+    scanned = plan.scanned_bytes()
+    stream = torch.cuda.current_stream()
+    plan.capture()
+    for _ in range(2):
+        plan.run_graph()
+    torch.cuda.synchronize()
+    plan.check_errors()
+    reps = 5
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        plan.run_graph()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    batched_ms = e0.elapsed_time(e1) / reps
+    _log(f"batched: {batched_ms:.3f} ms per batch")
+
+    # plain form: per (set, buffer) job one transpose and one launch per group,
+    # inputs resident on the device (16-byte row stride, as the transpose needs)
+    psets = [compiler.compile_pattern_set(g).load(0) for g in set_groups]
+    sptr = rt.current_stream_ptr(stream)
+    staged, basis, results, status = [], [], {}, {}
+    for b in buffers:
+        n = int(b.size)
+        stride = (n + 15) // 16 * 16
+        x = torch.zeros(max(stride, 16), dtype=torch.uint8, device=dev)
+        x[:n] = torch.from_numpy(b).to(dev)
+        staged.append(x)
+        basis.append(torch.empty((8, words_for(n)), dtype=torch.uint64, device=dev))
+    for (si, bi) in jobs:
+        W = words_for(int(buffers[bi].size))
+        results[(si, bi)] = torch.empty((psets[si].n_groups, W), dtype=torch.uint64, device=dev)
+        status[(si, bi)] = psets[si].alloc_status(1, W)
+
+    def plain_pass():
+        for (si, bi) in jobs:
+            ks = psets[si]
+            n = int(buffers[bi].size)
+            W = words_for(n)
+            rt.transpose(staged[bi].data_ptr(), n, staged[bi].numel(), basis[bi].data_ptr(), W, 1, sptr)
+            r, st = results[(si, bi)], status[(si, bi)]
+            for g in range(ks.n_groups):
+                ks.launch(g, basis[bi].data_ptr(), r.data_ptr() + g * W * 8, st.data_ptr() + g * st.size(2) * 4,
+                          W, sptr)
+
+    plain_pass()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    plain_pass()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    for ks in psets:
+        ks.check_errors()
+    plain_ms = e0.elapsed_time(e1)
+    _log(f"plain per-job: {plain_ms:.3f} ms for {len(jobs)} jobs")
+    # the two forms agree bit for bit on a sample of jobs
+    rng = np.random.default_rng(0)
+    for k in rng.choice(len(jobs), size=min(64, len(jobs)), replace=False):
+        si, bi = jobs[int(k)]
+        if not torch.equal(plan.results(si, bi), results[(si, bi)]):
+            raise AssertionError(f"batched and plain results differ for set {si} buffer {bi}")
+    plan.destroy()
+    return {
+        "workload": "batched_uneven",
+        "sets_regexes": [len(s) for s in sets],
+        "sets_groups": [len(g) for g in set_groups],
+        "buffers": len(buffers),
+        "jobs": len(jobs),
+        "scanned_bytes": int(scanned),
+        "batched_ms": round(batched_ms, 4),
+        "batched_gbs": round(scanned / (batched_ms / 1e3) / 1e9, 3),
+        "batched_pct_hbm_peak": round(100 * scanned / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "plain_ms": round(plain_ms, 4),
+        "plain_gbs": round(scanned / (plain_ms / 1e3) / 1e9, 3),
+        "plain_launches": len(jobs) + sum(psets[si].n_groups for si, _ in jobs),
+        "speedup": round(plain_ms / batched_ms, 3),
+        "form": "batched = one graph replay (1 transpose + 1 persistent launch per group); "
+                "plain = 1 transpose + 1 launch per group per job",
+    }
+
+
+def served_measurement(args):
+    """Continuous-batching server vs synchronous per-job plain launches on
+    the same Poisson arrival schedules (see the module docstring)."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    from bitgen_amd import compiler, workloads
+    from bitgen_amd.batch import BatchPatternSet
+    from bitgen_amd.serve import MatchServer, run_plain_job
+
+    sizes = (32, 64, 128, 256, 512, 1024, 1536, 2048)
+    t0 = time.perf_counter()
+    set_groups = [compiler.group_regex_in_list(workloads.ids_regexes(n, 100 + i), workloads.groups_for(n))
+                  for i, n in enumerate(sizes)]
+    bsets = {i: BatchPatternSet(g).load(0) for i, g in enumerate(set_groups)}
+    psets = {i: compiler.compile_pattern_set(g).load(0) for i, g in enumerate(set_groups)}
+    _log(f"served pattern sets loaded: {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
+    dev = torch.device("cuda", 0)
+    out = {"workload": "served", "sets_regexes": list(sizes), "sets_groups": [len(g) for g in set_groups],
+           "hbm_cap_bytes": 200 * 10**9, "rates": []}
+    for rate in [float(r) for r in args.served_rates.split(",") if r]:
+        arrivals, set_idx, bufs = workloads.serving(args.served_jobs, rate, seed=11 + int(rate))
+        nbytes = int(sum(b.size for b in bufs))
+        # warm the server's paths once
+        warm = MatchServer(bsets).start()
+        warm.submit(0, bufs[0]).result(timeout=60)
+        warm.stop()
+
+        srv = MatchServer(bsets, hbm_cap_bytes=200 * 10**9).start()
+        futs = [None] * len(bufs)
+
+        def producer():
+            t_start = time.perf_counter()
+            for i, (a, s, b) in enumerate(zip(arrivals, set_idx, bufs)):
+                delay = t_start + a - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                futs[i] = srv.submit(int(s), b)
+
+        th = threading.Thread(target=producer)
+        t_first = time.perf_counter()
+        th.start()
+        th.join()
+        for f in futs:
+            f.result(timeout=600)
+        srv.stop()
+        lat = srv.latencies()
+        t_last = max(j.done for j in srv.completed)
+        served_gbs = nbytes / (t_last - t_first) / 1e9
+
+        # per-job plain launches, FIFO on one host thread, same arrivals
+        lat_plain = np.empty(len(bufs))
+        run_plain_job(psets[0], bufs[0], dev)
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for i, (a, s, b) in enumerate(zip(arrivals, set_idx, bufs)):
+            now = time.perf_counter()
+            if now < t_start + a:
+                time.sleep(t_start + a - now)
+            run_plain_job(psets[int(s)], b, dev)
+            lat_plain[i] = time.perf_counter() - (t_start + a)
+        plain_gbs = nbytes / (time.perf_counter() - t_start) / 1e9
+        row = {
+            "rate_per_s": rate,
+            "jobs": len(bufs),
+            "bytes": nbytes,
+            "served_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "served_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "served_gbs": round(served_gbs, 4),
+            "served_batches": srv.batches,
+            "plain_p50_ms": round(float(np.percentile(lat_plain, 50)) * 1e3, 3),
+            "plain_p99_ms": round(float(np.percentile(lat_plain, 99)) * 1e3, 3),
+            "plain_gbs": round(plain_gbs, 4),
+        }
+        _log(f"served at {rate:.0f}/s: {row}")
+        out["rates"].append(row)
+    for ks in psets.values():
+        ks.free()
+    for s in bsets.values():
+        s.free()
+    return out
 
 
 def profiled_traffic(args):
@@ -242,6 +452,11 @@ def main():
     }
     if count is not None:
         out["match_count"] = count
+    ks.free()
+    if not args.no_batched:
+        out["batched"] = batched_measurement(args)
+    if not args.no_served:
+        out["served"] = served_measurement(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(groups, buf, args.cpu_sample_mib * 2**20)
     print(json.dumps(out), flush=True)

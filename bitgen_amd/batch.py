@@ -23,8 +23,8 @@ import os
 import numpy as np
 
 from .backend.hip import runtime as rt
-from .codegen.gen_hip import GroupProgram, batch_kernel_name, batch_source, compile_sources
-from .compiler import DEFAULT_OPTIONS, group_programs, _set_dir
+from .codegen.gen_hip import batch_kernel_name, batch_source, compile_sources
+from .compiler import DEFAULT_OPTIONS, group_programs, load_cached_set, program_for, store_cached_set, _set_dir
 
 WAVE_WORDS = 64
 
@@ -50,12 +50,19 @@ class BatchPatternSet:
         self.groups = [list(g) for g in groups]
         self.options = o
         self.block_threads = block_threads
+        self.modules = None
+        self.kernels = None
+        d = _set_dir(self.groups, dict(o, form="batch", block=block_threads))
+        cached = load_cached_set(d, "batch.json")
+        if cached is not None:
+            self.metas = cached
+            self.hsacos = [m["hsaco"] for m in cached]
+            return
         programs = []
         for gid, regs in enumerate(self.groups):
             _, _, post, vnm = group_programs(regs, o)
-            programs.append(GroupProgram(gid, post, vnm, regs, lookback=o["lookback"]))
+            programs.append(program_for(gid, post, vnm, regs, o))
         self.metas = [p.meta() for p in programs]
-        d = _set_dir(self.groups, dict(o, form="batch", block=block_threads))
         sources = {}
         for p in programs:
             path = os.path.join(d, f"batch_{p.gid}.hip")
@@ -65,8 +72,9 @@ class BatchPatternSet:
             sources[path] = src
         hs = compile_sources(sources)
         self.hsacos = [os.path.abspath(hs[os.path.join(d, f"batch_{p.gid}.hip")]) for p in programs]
-        self.modules = None
-        self.kernels = None
+        for m, h in zip(self.metas, self.hsacos):
+            m["hsaco"] = h
+        store_cached_set(d, "batch.json", self.metas)
 
     @property
     def n_groups(self):

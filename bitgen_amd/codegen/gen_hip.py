@@ -83,10 +83,15 @@ def _cname(v):
 class GroupProgram:
     """Analysis + HIP source of one group program."""
 
-    def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate"):
+    def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
-        predecessor's status)."""
+        predecessor's status).  expect_zero: mark every zero test as likely
+        taken, so the compiler lays the skipped code out of line and the
+        common path falls through.  pipeline: compute the first stage of the
+        next regex (the part before its first zero test) ahead of the
+        current regex's first zero test, so its instructions fill the
+        latencies of the test and of the lane moves."""
         if lookback not in ("speculate", "wait"):
             raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
         self.gid = gid
@@ -94,6 +99,8 @@ class GroupProgram:
         self.vnm = vnm
         self.regexes = regexes or []
         self.lookback = lookback
+        self.expect_zero = bool(expect_zero)
+        self.pipeline = bool(pipeline)
         self.kernel_name = f"bg_group_{gid}"
         self._check()
         self._halo()
@@ -265,8 +272,13 @@ class GroupProgram:
         header = [d for k, i in enumerate(self.insts[:first_skip]) if k not in absorbed for d in i.defs()]
         later = {u for k, i in enumerate(self.insts) if k >= first_skip and k not in absorbed for u in eff_uses(k, i)}
         fence_vars = [v for v in dict.fromkeys(header) if v in later]
+        hoist_at, hoisted = self._pipeline_plan(trees, absorbed) if self.pipeline else ({}, set())
+        chunks = [[] for _ in self.insts]
+        body_lines = lines
         open_regions = set()
         for idx, inst in enumerate(self.insts):
+            lines = chunks[idx]
+            add = lines.append
             if speculative and idx == body_start:
                 add("  for (;;) {")
                 zeros = [f"{_cname(v)} = 0;" for v in body_vars]
@@ -298,7 +310,10 @@ class GroupProgram:
                     add(f"  {{ int bg_unused = -1; {_cname(inst.ret)} = {fn}<{L}, S, {k}>("
                         f"{self.val(inst.operands[0])}, {self.val(inst.operands[1])}, tc, -1, bg_unused); }}")
             elif t == BsInstType.SKIPZERO:
-                add(f"  if (bg_wave_zero({self.val(inst.operands[0])})) goto L{inst.label};")
+                test = f"bg_wave_zero({self.val(inst.operands[0])})"
+                if self.expect_zero:
+                    test = f"__builtin_expect({test}, 1)"
+                add(f"  if ({test}) goto L{inst.label};")
                 open_regions.add(inst.label)
             elif t == BsInstType.LABEL:
                 add(f"L{inst.label}:;")
@@ -313,6 +328,25 @@ class GroupProgram:
                 pass
             else:
                 raise HipCodegenError(f"unsupported instruction {inst}")
+        # assemble: hoisted first stages go right before the zero test they
+        # were scheduled against (hoisting never crosses a pass-loop header:
+        # every target is a zero test, and the loop starts at or before the
+        # first one)
+        lines = body_lines
+        add = lines.append
+        for idx in range(len(self.insts)):
+            if idx in hoist_at:
+                head = chunks[idx]
+                # the pass-loop header, if it opens here, stays in front
+                lead = []
+                while head and not head[0].lstrip().startswith("if ("):
+                    lead.append(head.pop(0))
+                lines.extend(lead)
+                for h in hoist_at[idx]:
+                    lines.extend(chunks[h])
+                lines.extend(head)
+            elif idx not in hoisted:
+                lines.extend(chunks[idx])
         if speculative:
             add("bg_pass_end:;")
             add(f"  if (bg_settle<{self.n_carry}>(tc)) break;")
@@ -322,6 +356,71 @@ class GroupProgram:
         add("  if (inb && lane >= lo && lane < hi) result[w] = out;")
         add("}")
         return "\n".join(lines) + "\n"
+
+    def _pipeline_plan(self, trees, absorbed):
+        """Which first-stage instructions move up one regex.
+
+        Regexes are the stretches between labels that close every open skip
+        region.  For consecutive regexes j and j+1, the pure instructions of
+        j+1 before its first zero test (bitwise trees, shifts) move to just
+        before j's first zero test when every input is already defined there:
+        a constant, a single-definition value defined before that test, or
+        another moved instruction.  A moved value is defined once, so it is
+        the same wherever it is computed.  Returns ({zero-test index: [moved
+        indices in program order]}, set of moved indices)."""
+        insts = self.insts
+        ndefs = {}
+        def_at = {}
+        for k, inst in enumerate(insts):
+            for d in inst.defs():
+                ndefs[d] = ndefs.get(d, 0) + 1
+                def_at[d] = k
+        # segments: (start index, first top-level zero test or None, end index)
+        segs, start, open_, first = [], 0, set(), None
+        for k, inst in enumerate(insts):
+            if inst.type == BsInstType.SKIPZERO:
+                if not open_ and first is None:
+                    first = k
+                open_.add(inst.label)
+            elif inst.type == BsInstType.LABEL:
+                open_.discard(inst.label)
+                if not open_:
+                    segs.append((start, first, k))
+                    start, first = k + 1, None
+        segs.append((start, first, len(insts) - 1))
+
+        def uses(k):
+            if k in trees:
+                return _leaves(trees[k], [])
+            return insts[k].uses()
+
+        hoist_at, hoisted = {}, set()
+        for (s0, f0, _e0), (s1, f1, _e1) in zip(segs, segs[1:]):
+            if f0 is None or f1 is None:
+                continue
+            moved = []
+            moved_defs = set()
+            for k in range(s1, f1):
+                inst = insts[k]
+                if k in absorbed:
+                    continue
+                pure = (inst.type in _BITWISE and k in trees) or inst.type == BsInstType.ADVANCE
+                if not pure or inst.ret is None or ndefs.get(inst.ret) != 1:
+                    continue
+                ok = True
+                for u in uses(k):
+                    if self._const(u) is not None or u in moved_defs:
+                        continue
+                    if ndefs.get(u) != 1 or def_at[u] >= f0:
+                        ok = False
+                        break
+                if ok:
+                    moved.append(k)
+                    moved_defs.add(inst.ret)
+            if moved:
+                hoist_at[f0] = moved
+                hoisted.update(moved)
+        return hoist_at, hoisted
 
     def _fuse(self):
         """Bitwise expression trees for v_bitop3_b32 (three-input lookup
@@ -547,7 +646,7 @@ def batch_source(program, block_threads=256):
     ])
 
 
-def _hsaco_cache_dir():
+def hsaco_cache_dir():
     d = cfg.get_config("hsaco_cache_dir") or os.path.join(cfg.get_config("cache_dir", ".cache_bitgen"), "hsaco")
     os.makedirs(d, exist_ok=True)
     return d
@@ -555,7 +654,7 @@ def _hsaco_cache_dir():
 
 def compile_sources(sources, workers=None):
     """Compile {path: source} to cached hsaco files; returns {path: hsaco}."""
-    cache = _hsaco_cache_dir()
+    cache = hsaco_cache_dir()
     jobs = {}
     out = {}
     for path, src in sources.items():

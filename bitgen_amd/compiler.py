@@ -34,6 +34,11 @@ DEFAULT_OPTIONS = {
     # resolve once per tile, replay when wrong) or "wait" (every carry op
     # waits for its predecessor tile)
     "lookback": "speculate",
+    # instruction scheduling of the generated group programs: zero tests
+    # marked likely taken (skipped code laid out of line), and each regex's
+    # first stage computed ahead of the previous regex's first zero test
+    "sched_expect": 1,
+    "sched_pipeline": 0,
 }
 
 
@@ -100,6 +105,12 @@ def group_programs(regexes, options=None):
     return pre, pre_vnm, post, post_vnm
 
 
+def program_for(gid, insts, vnm, regexes, options):
+    """GroupProgram of one group's post-pass IR under the compile options."""
+    return GroupProgram(gid, insts, vnm, regexes, lookback=options["lookback"],
+                        expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]))
+
+
 @dataclass
 class CompiledPatternSet:
     groups: List[List[str]]
@@ -113,6 +124,31 @@ class CompiledPatternSet:
         return HipGroupSet(self.metas, self.base_dir, device)
 
 
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_code_digest = None
+
+
+def code_digest():
+    """Digest of every source that shapes a compiled pattern set: the regex
+    front end, parser, IR, passes, code generator, device headers and build
+    flags.  A cached set is reused only under the same digest."""
+    global _code_digest
+    if _code_digest is None:
+        h = hashlib.sha256()
+        paths = [os.path.join(_PKG, "compiler.py"), os.path.join(_PKG, "csrc", "build.py"),
+                 os.path.join(_PKG, "template", "hip", "bitgen_device.h"),
+                 os.path.join(os.path.dirname(_PKG), "include", "bitgen_hip.h")]
+        for sub in ("tool", "bitparser", "inst", "passes", "codegen"):
+            d = os.path.join(_PKG, sub)
+            paths += [os.path.join(d, f) for f in sorted(os.listdir(d)) if f.endswith(".py")]
+        for path in paths:
+            h.update(path.encode())
+            with open(path, "rb") as f:
+                h.update(f.read())
+        _code_digest = h.hexdigest()[:24]
+    return _code_digest
+
+
 def _set_dir(groups, options):
     h = hashlib.sha256(json.dumps([groups, sorted(options.items())]).encode()).hexdigest()[:20]
     root = cfg.get_config("cache_dir", ".cache_bitgen")
@@ -121,16 +157,49 @@ def _set_dir(groups, options):
     return d
 
 
+def load_cached_set(d, name):
+    """Metas of a set compiled earlier under the same code digest whose code
+    objects are all still present, else None."""
+    from .codegen.gen_hip import hsaco_cache_dir
+
+    path = os.path.join(d, name)
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("digest") != code_digest():
+        return None
+    metas = []
+    for m in rec["metas"]:
+        m = dict(m, hsaco=os.path.abspath(os.path.join(hsaco_cache_dir(), m["hsaco"])))
+        if not os.path.exists(m["hsaco"]):
+            return None
+        metas.append(m)
+    return metas
+
+
+def store_cached_set(d, name, metas):
+    rec = {"digest": code_digest(), "metas": [dict(m, hsaco=os.path.basename(m["hsaco"])) for m in metas]}
+    tmp = os.path.join(d, name + ".tmp")
+    with open(tmp, "w") as f:
+        json.dump(rec, f)
+    os.replace(tmp, os.path.join(d, name))
+
+
 def compile_pattern_set(groups, options=None, workers=None):
     """Compile regex groups (list of lists) into gfx950 code objects."""
     o = dict(DEFAULT_OPTIONS)
     o.update(options or {})
     groups = [list(g) for g in groups]
     d = _set_dir(groups, o)
+    cached = load_cached_set(d, "plain.json")
+    if cached is not None:
+        return CompiledPatternSet(groups, cached, d, o)
     programs, sources = [], {}
     for gid, regs in enumerate(groups):
         _, _, post, post_vnm = group_programs(regs, o)
-        prog = GroupProgram(gid, post, post_vnm, regs, lookback=o["lookback"])
+        prog = program_for(gid, post, post_vnm, regs, o)
         programs.append(prog)
         path = os.path.join(d, f"kernel_{gid}.hip")
         src = prog.source()
@@ -143,4 +212,5 @@ def compile_pattern_set(groups, options=None, workers=None):
         m = prog.meta()
         m["hsaco"] = os.path.abspath(hsacos[os.path.join(d, f"kernel_{prog.gid}.hip")])
         metas.append(m)
+    store_cached_set(d, "plain.json", metas)
     return CompiledPatternSet(groups, metas, d, o)

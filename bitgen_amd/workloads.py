@@ -17,6 +17,7 @@ Workloads (sizes as named; every generator is deterministic in its seed):
 * ``serving``: 8 resident ``ids``-mix sets and Poisson arrivals of jobs with
   log-normal buffers (median 4 KiB, max 64 MiB).
 """
+import functools
 import random
 
 import numpy as np
@@ -105,10 +106,19 @@ def genomic_regexes(n, seed):
 
 
 # ---- sampling from a regex's language ----------------------------------------
+@functools.lru_cache(maxsize=None)
+def _cc_choices(mask):
+    return [b for b in range(256) if (mask >> b) & 1 and b != LF]
+
+
+@functools.lru_cache(maxsize=4096)
+def _parsed(regex):
+    return parse_regex(regex)
+
+
 def _sample(node, rng, out):
     if isinstance(node, CC):
-        choices = [b for b in range(256) if (node.mask >> b) & 1 and b != LF]
-        out.append(rng.choice(choices))
+        out.append(rng.choice(_cc_choices(node.mask)))
     elif isinstance(node, Seq):
         for it in node.items:
             _sample(it, rng, out)
@@ -126,7 +136,7 @@ def _sample(node, rng, out):
 
 def sample_match(regex, rng):
     out = []
-    _sample(parse_regex(regex), rng, out)
+    _sample(_parsed(regex), rng, out)
     return bytes(out)
 
 
