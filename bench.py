@@ -54,7 +54,9 @@ def parse():
     ap.add_argument("--grid-blocks", type=int, default=0, help="persistent blocks per launch (0 = all that fit)")
     ap.add_argument("--take", type=int, default=0, help="tiles per queue take (0 = runtime default)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
-                    help="compiler option override (e.g. pass_short_circuit_interval=2)")
+                    help="compiler option override (e.g. pass_short_circuit_interval=2); skips the autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuner's")
+    ap.add_argument("--tune-sample-mib", type=int, default=16, help="sample the autotuner times candidates on")
     ap.add_argument("--cpu-sample-mib", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check the GPU result against the oracle on a slice")
@@ -62,7 +64,7 @@ def parse():
     ap.add_argument("--batched-mib", type=int, default=512, help="total bytes of the batched workload's buffers")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--served-jobs", type=int, default=3000, help="jobs per arrival rate")
-    ap.add_argument("--served-rates", default="5000,20000", help="Poisson arrival rates (jobs/s), comma separated")
+    ap.add_argument("--served-rates", default="2000,10000,50000", help="Poisson arrival rates (jobs/s), comma separated")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (three times) and exit: the reference kernel "
@@ -137,7 +139,8 @@ def batched_measurement(args):
     _log(f"batched workload ready: {len(buffers)} buffers, {sum(b.size for b in buffers)} bytes, sets "
          f"{[len(s) for s in sets]} regexes in {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
     dev = torch.device("cuda", 0)
-    bsets = [BatchPatternSet(g).load(0) for g in set_groups]
+    opts = dict(getattr(args, "tuned_options", {}) or {})
+    bsets = [BatchPatternSet(g, opts).load(0) for g in set_groups]
     plan = BatchPlan(bsets, buffers, jobs)
     scanned = plan.scanned_bytes()
     stream = torch.cuda.current_stream()
@@ -159,7 +162,9 @@ def batched_measurement(args):
 
     # plain form: per (set, buffer) job one transpose and one launch per group,
     # inputs resident on the device (16-byte row stride, as the transpose needs)
-    psets = [compiler.compile_pattern_set(g).load(0) for g in set_groups]
+    psets = [compiler.compile_pattern_set(g, opts).load(0) for g in set_groups]
+    for ks in psets:
+        ks.n_queues = args.queues
     sptr = rt.current_stream_ptr(stream)
     staged, basis, results, status = [], [], {}, {}
     for b in buffers:
@@ -181,9 +186,9 @@ def batched_measurement(args):
             W = words_for(n)
             rt.transpose(staged[bi].data_ptr(), n, staged[bi].numel(), basis[bi].data_ptr(), W, 1, sptr)
             r, st = results[(si, bi)], status[(si, bi)]
-            for g in range(ks.n_groups):
-                ks.launch(g, basis[bi].data_ptr(), r.data_ptr() + g * W * 8, st.data_ptr() + g * st.size(2) * 4,
-                          W, sptr)
+            for k in range(ks.n_kernels):
+                ks.launch(k, basis[bi].data_ptr(), r.data_ptr() + ks.first_group[k] * W * 8,
+                          st.data_ptr() + k * st.size(2) * 4, W, sptr)
 
     plain_pass()
     torch.cuda.synchronize()
@@ -214,7 +219,8 @@ def batched_measurement(args):
         "batched_pct_hbm_peak": round(100 * scanned / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "plain_ms": round(plain_ms, 4),
         "plain_gbs": round(scanned / (plain_ms / 1e3) / 1e9, 3),
-        "plain_launches": len(jobs) + sum(psets[si].n_groups for si, _ in jobs),
+        "plain_launches": len(jobs) + sum(psets[si].n_kernels for si, _ in jobs),
+        "launch_shape": {"groups_per_kernel": opts.get("groups_per_kernel", 1), "min_waves": opts.get("min_waves", 0)},
         "speedup": round(plain_ms / batched_ms, 3),
         "form": "batched = one graph replay (1 transpose + 1 persistent launch per group); "
                 "plain = 1 transpose + 1 launch per group per job",
@@ -237,8 +243,9 @@ def served_measurement(args):
     t0 = time.perf_counter()
     set_groups = [compiler.group_regex_in_list(workloads.ids_regexes(n, 100 + i), workloads.groups_for(n))
                   for i, n in enumerate(sizes)]
-    bsets = {i: BatchPatternSet(g).load(0) for i, g in enumerate(set_groups)}
-    psets = {i: compiler.compile_pattern_set(g).load(0) for i, g in enumerate(set_groups)}
+    opts = dict(getattr(args, "tuned_options", {}) or {})
+    bsets = {i: BatchPatternSet(g, opts).load(0) for i, g in enumerate(set_groups)}
+    psets = {i: compiler.compile_pattern_set(g, opts).load(0) for i, g in enumerate(set_groups)}
     _log(f"served pattern sets loaded: {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
     dev = torch.device("cuda", 0)
     out = {"workload": "served", "sets_regexes": list(sizes), "sets_groups": [len(g) for g in set_groups],
@@ -285,6 +292,7 @@ def served_measurement(args):
             run_plain_job(psets[int(s)], b, dev)
             lat_plain[i] = time.perf_counter() - (t_start + a)
         plain_gbs = nbytes / (time.perf_counter() - t_start) / 1e9
+        blog = np.array(srv.batch_log) if srv.batch_log else np.zeros((1, 3))
         row = {
             "rate_per_s": rate,
             "jobs": len(bufs),
@@ -293,6 +301,9 @@ def served_measurement(args):
             "served_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "served_gbs": round(served_gbs, 4),
             "served_batches": srv.batches,
+            "served_jobs_per_batch_mean": round(float(blog[:, 0].mean()), 2),
+            "served_host_stage_ms_p50_p99": [round(float(np.percentile(blog[:, 1], q)) * 1e3, 3) for q in (50, 99)],
+            "served_batch_device_ms_p50_p99": [round(float(np.percentile(blog[:, 2], q)) * 1e3, 3) for q in (50, 99)],
             "plain_p50_ms": round(float(np.percentile(lat_plain, 50)) * 1e3, 3),
             "plain_p99_ms": round(float(np.percentile(lat_plain, 99)) * 1e3, 3),
             "plain_gbs": round(plain_gbs, 4),
@@ -347,8 +358,20 @@ def main():
     for kv in args.option:
         k, v = kv.split("=", 1)
         options[k] = int(v) if v.lstrip("-").isdigit() else v
+    tuning = None
+    if not options and not args.no_tune:
+        from bitgen_amd import autotune
+
+        tuning = autotune.tune(groups, buf[:args.tune_sample_mib * 2**20])
+        options = dict(tuning["options"])
+        if not args.queues:
+            args.queues = tuning["queues"]
+        _log(f"autotuner: {tuning['options']} queues {tuning['queues']} "
+             f"({[(c['groups_per_kernel'], c['min_waves'], c['ms']) for c in tuning['candidates']]})")
+    args.tuned_options = options
     cs = compiler.compile_pattern_set(groups, options)
-    print(f"[bench] compiled {len(groups)} group kernels ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    print(f"[bench] compiled {len(groups)} groups into {len(cs.metas)} kernels ({time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
     dev = torch.device("cuda", 0)
     ks = cs.load(0)
     ks.n_queues = args.queues
@@ -367,7 +390,7 @@ def main():
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream()
-    G = ks.n_groups
+    K = ks.n_kernels
 
     from bitgen_amd.backend.hip import runtime as rt
 
@@ -379,11 +402,11 @@ def main():
         transpose_bytes(staged, n_bytes, out=basis)
         if events is not None:
             events[1].record(stream)
-        for g in range(G):
-            ks.launch(g, basis.data_ptr(), result.data_ptr() + g * W * 8,
-                      status.data_ptr() + g * status.size(2) * 4, W, sptr)
+        for k in range(K):
+            ks.launch(k, basis.data_ptr(), result.data_ptr() + ks.first_group[k] * W * 8,
+                      status.data_ptr() + k * status.size(2) * 4, W, sptr)
             if events is not None:
-                events[2 + g].record(stream)
+                events[2 + k].record(stream)
 
     for _ in range(args.warmup):
         step()
@@ -391,7 +414,7 @@ def main():
     ks.check_errors()
     print("[bench] warm-up done", file=sys.stderr, flush=True)
 
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2 + G)] for _ in range(args.steps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2 + K)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for s in range(args.steps):
@@ -401,14 +424,15 @@ def main():
     ks.check_errors()
 
     trans_ms = [e[0].elapsed_time(e[1]) for e in ev]
-    group_ms = [[ev[s][1 + g].elapsed_time(ev[s][2 + g]) for g in range(G)] for s in range(args.steps)]
-    step_ms = [e[0].elapsed_time(e[1 + G]) for e in ev]
+    kernel_ms = [[ev[s][1 + k].elapsed_time(ev[s][2 + k]) for k in range(K)] for s in range(args.steps)]
+    step_ms = [e[0].elapsed_time(e[1 + K]) for e in ev]
     ms_per_step = float(np.median(step_ms))
     value = n_bytes / (ms_per_step / 1e3) / 1e9
-    mean_group_ms = float(np.mean(group_ms))
-    alg_bytes_launch = n_bytes + W * 8  # basis read + match stream write
-    achieved = alg_bytes_launch / (mean_group_ms / 1e3) / 1e9
-    per_group = [float(np.median([group_ms[s][g] for s in range(args.steps)])) for g in range(G)]
+    # algorithmic bytes of a kernel launch: basis read (N bytes) + its groups'
+    # match streams written (8 W bytes each)
+    alg_bytes = [n_bytes + m.get("n_out", 1) * W * 8 for m in ks.metas]
+    achieved = sum(alg_bytes) / (float(np.mean([sum(r) for r in kernel_ms])) / 1e3) / 1e9
+    per_kernel = [float(np.median([kernel_ms[s][k] for s in range(args.steps)])) for k in range(K)]
 
     count = None
     if args.check:
@@ -432,15 +456,21 @@ def main():
         "config": {
             "workload": "ids_plain",
             "regexes": len(regexes),
-            "groups": G,
+            "groups": ks.n_groups,
+            "kernels": K,
+            "groups_per_kernel": cs.options["groups_per_kernel"],
             "buffer_bytes": n_bytes,
-            "form": "plain (one launch per group per buffer)",
+            "form": "plain (one transpose + one launch per kernel for the buffer)",
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 4),
         "transpose_ms": round(float(np.median(trans_ms)), 4),
-        "group_kernel_ms": [round(v, 4) for v in per_group],
+        "kernel_ms": [round(v, 4) for v in per_kernel],
         "wall_s": round(wall, 3),
+        "autotune": ({"chosen": {"groups_per_kernel": cs.options["groups_per_kernel"],
+                                 "min_waves": cs.options["min_waves"], "queues": args.queues},
+                      "sample_bytes": tuning["sample_bytes"], "candidates": tuning["candidates"]}
+                     if tuning else None),
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 3),

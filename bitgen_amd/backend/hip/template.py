@@ -7,8 +7,8 @@ tmp_streams, kernel_handle, run, dyn_stats)`` where
 
 * ``result``: torch.uint64 ``[1, M, G, W]`` match-end streams, LSB-first words
   (``to_reference_layout`` gives the reference's u32 ``[1, M, G, 2W]``);
-* ``tmp_streams``: the carry look-back status words ``[M, G, T*C]`` (uint32
-  stored as int32) -- the fused hip kernels spill no streams;
+* ``tmp_streams``: the carry look-back status words ``[M, K, T*C]`` (uint32
+  stored as int32, K kernels) -- the fused hip kernels spill no streams;
 * ``kernel_handle``: a ``HipGroupSet`` (``.free()`` unloads the modules);
 * ``run(basic_stream_tensor)``: launches every (input, group) pair on the
   current torch stream and synchronises it, like CUDA_Kernel.run
@@ -30,7 +30,9 @@ def _torch():
 
 
 class HipGroupSet:
-    """Loaded group kernels of one pattern set plus their launch state."""
+    """Loaded kernels of one pattern set plus their launch state.  A kernel
+    computes one group, or several consecutive groups merged into one
+    program (meta "groups"); results are always per group."""
 
     def __init__(self, metas, base_dir, device=0):
         torch = _torch()
@@ -44,17 +46,27 @@ class HipGroupSet:
             mod = rt.Module(path)
             self.modules.append(mod)
             self.kernels.append(mod.kernel(m["kernel"]))
+        # first group of each kernel (metas without "groups" hold one group)
+        self.first_group = []
+        g = 0
+        for m in metas:
+            self.first_group.append(m.get("first_group", g))
+            g = self.first_group[-1] + m.get("n_out", 1)
         dev = torch.device("cuda", device)
-        # one control block per group (tile queues, error flag, launch epoch):
-        # launches of a group are ordered on one stream, groups never share one
+        # one control block per kernel (tile queues, error flag, launch epoch):
+        # launches of a kernel are ordered on one stream, kernels never share one
         self.ctl = torch.zeros((len(metas), rt.CTL_WORDS), dtype=torch.int32, device=dev)
         self.n_queues = 0  # 0 = the runtime's default
         self.take = 0  # tiles per queue take, 0 = the runtime's default
         self.grid_blocks = 0  # 0 = as many blocks as fit on the device
 
     @property
-    def n_groups(self):
+    def n_kernels(self):
         return len(self.metas)
+
+    @property
+    def n_groups(self):
+        return sum(m.get("n_out", 1) for m in self.metas)
 
     @staticmethod
     def n_tiles(meta, n_words):
@@ -67,54 +79,59 @@ class HipGroupSet:
         return max(self.n_tiles(m, n_words) * max(1, m["n_carry"]) for m in self.metas)
 
     def alloc_status(self, n_inputs, n_words):
-        """Zeroed look-back status words for n_inputs buffers of n_words words.
-        A status buffer belongs to this group set: its words are tagged with
-        the epochs of this set's control blocks."""
+        """Zeroed look-back status words [n_inputs, kernels, words] for buffers
+        of n_words words.  A status buffer belongs to this group set: its words
+        are tagged with the epochs of this set's control blocks."""
         torch = _torch()
-        return torch.zeros((n_inputs, self.n_groups, self.status_words(n_words)), dtype=torch.int32,
+        return torch.zeros((n_inputs, self.n_kernels, self.status_words(n_words)), dtype=torch.int32,
                            device=torch.device("cuda", self.device))
 
-    def launch(self, g, basis_ptr, result_ptr, status_ptr, n_words, stream):
-        """Enqueue group g over one buffer (asynchronous)."""
-        meta = self.metas[g]
+    def launch(self, k, basis_ptr, result_ptr, status_ptr, n_words, stream):
+        """Enqueue kernel k over one buffer (asynchronous).  ``result_ptr`` is
+        the stream of the kernel's first group; its j-th group's stream
+        follows n_words words later for each j."""
+        meta = self.metas[k]
         args = rt.GroupArgs()
         args.basis = basis_ptr
         args.result = result_ptr
         args.status = status_ptr
-        args.ctl = self.ctl[g].data_ptr()
+        args.ctl = self.ctl[k].data_ptr()
         args.n_words = n_words
         args.n_tiles = self.n_tiles(meta, n_words)
         args.n_queues = self.n_queues
         args.n_carry = max(1, meta["n_carry"])
         args.take = self.take
-        rt.group_launch(self.kernels[g], args, stream, self.grid_blocks)
+        rt.group_launch(self.kernels[k], args, stream, self.grid_blocks)
 
     def run(self, basis, result, status, stream=None):
-        """basis u64 [S=1, M, 8, W]; result u64 [1, M, G, W]; status int32 [M, G, *]."""
+        """basis u64 [S=1, M, 8, W]; result u64 [1, M, G, W]; status int32 [M, kernels, *]."""
         torch = _torch()
         if basis.dim() != 4 or basis.dtype != torch.uint64 or basis.size(2) != 8:
             raise ValueError(f"basis must be uint64 [1, M, 8, W], got {tuple(basis.shape)} {basis.dtype}")
         if basis.size(0) != 1:
             raise NotImplementedError("split inputs (dimension 0 > 1) are not supported")
         M, W = basis.size(1), basis.size(3)
-        if tuple(result.shape) != (1, M, self.n_groups, W) or result.dtype != torch.uint64:
-            raise ValueError(f"result must be uint64 [1, {M}, {self.n_groups}, {W}]")
+        G, K = self.n_groups, self.n_kernels
+        if tuple(result.shape) != (1, M, G, W) or result.dtype != torch.uint64:
+            raise ValueError(f"result must be uint64 [1, {M}, {G}, {W}]")
+        if status.dim() != 3 or status.size(0) != M or status.size(1) != K:
+            raise ValueError(f"status must be int32 [{M}, {K}, words] (alloc_status)")
         if not (basis.is_contiguous() and result.is_contiguous() and status.is_contiguous()):
             raise ValueError("tensors must be contiguous")
         s = rt.current_stream_ptr(stream)
         for m in range(M):
             bptr = basis.data_ptr() + m * 8 * W * 8
-            for g in range(self.n_groups):
-                rptr = result.data_ptr() + (m * self.n_groups + g) * W * 8
-                sptr = status.data_ptr() + (m * self.n_groups + g) * status.size(2) * 4
-                self.launch(g, bptr, rptr, sptr, W, s)
+            for k in range(K):
+                rptr = result.data_ptr() + (m * G + self.first_group[k]) * W * 8
+                sptr = status.data_ptr() + (m * K + k) * status.size(2) * 4
+                self.launch(k, bptr, rptr, sptr, W, s)
 
     def check_errors(self):
         """Raise if any launch's carry look-back gave up (call after a sync)."""
         flags = self.ctl[:, rt.CTL_ERROR].cpu()
         if int(flags.abs().sum()) != 0:
-            bad = [g for g in range(self.n_groups) if int(flags[g]) != 0]
-            raise rt.HipRuntimeError(f"carry look-back timed out in groups {bad}; results are invalid")
+            bad = [k for k in range(self.n_kernels) if int(flags[k]) != 0]
+            raise rt.HipRuntimeError(f"carry look-back timed out in kernels {bad}; results are invalid")
 
     def info(self):
         return [mod.info(m["kernel"]) for mod, m in zip(self.modules, self.metas)]

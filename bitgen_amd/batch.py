@@ -3,19 +3,20 @@
 The reference launches one grid of (groups x inputs) blocks per pattern set
 over copies of one input (``gen_cuda.py:441-473``, ``run_bitgen.py:239-245``);
 uneven buffers need one launch each.  Here all buffers of a batch are
-transposed by one launch (a descriptor per buffer) and every (group, buffer)
+transposed by one launch (a descriptor per buffer) and every (kernel, buffer)
 job of a pattern set runs in one persistent launch of the set's batch kernel,
 whose waves take 4,096-position tiles from a device work queue
-(``tile_job``): one persistent launch per group of each set covers all
-buffers.  The sequence (one transpose launch + one batch launch per group) can
-be captured into a graph and replayed.
+(``tile_job``): one persistent launch per kernel of each set covers all
+buffers (a kernel computes one group, or several groups merged into one
+program).  The sequence (one transpose launch + one batch launch per kernel)
+can be captured into a graph and replayed.
 
 Layout (all device-resident, allocated once per plan):
   bytes   u8  [sum of 64-byte-aligned buffer lengths]
   basis   u64 [sum over buffers of 8 * W_b]
-  result  u64 [sum over jobs of W_b]       (per job: the group's match stream)
-  status  u32 [sum over jobs of T_j * C_g]
-  jobs    bg_job_t[n_jobs], tile_job u32[total tiles]    (per set and group)
+  result  u64 [sum over jobs of n_out * W_b]  (per job: its groups' streams)
+  status  u32 [sum over jobs of T_j * C_k]
+  jobs    bg_job_t[n_jobs], tile_job u32[total tiles]    (per set and kernel)
 """
 import ctypes
 import os
@@ -24,7 +25,7 @@ import numpy as np
 
 from .backend.hip import runtime as rt
 from .codegen.gen_hip import batch_kernel_name, batch_source, compile_sources
-from .compiler import DEFAULT_OPTIONS, group_programs, load_cached_set, program_for, store_cached_set, _set_dir
+from .compiler import DEFAULT_OPTIONS, kernel_meta, kernel_programs, load_cached_set, store_cached_set, _set_dir
 
 WAVE_WORDS = 64
 
@@ -41,8 +42,9 @@ def n_tiles(meta, W):
 
 
 class BatchPatternSet:
-    """The groups of one pattern set, each compiled into a persistent batch
-    kernel (separate code objects, compiled in parallel)."""
+    """The groups of one pattern set compiled into persistent batch kernels
+    (one kernel per ``groups_per_kernel`` consecutive groups, merged into one
+    program; separate code objects, compiled in parallel)."""
 
     def __init__(self, groups, options=None, block_threads=256):
         o = dict(DEFAULT_OPTIONS)
@@ -58,11 +60,8 @@ class BatchPatternSet:
             self.metas = cached
             self.hsacos = [m["hsaco"] for m in cached]
             return
-        programs = []
-        for gid, regs in enumerate(self.groups):
-            _, _, post, vnm = group_programs(regs, o)
-            programs.append(program_for(gid, post, vnm, regs, o))
-        self.metas = [p.meta() for p in programs]
+        programs = kernel_programs(self.groups, o)
+        self.metas = [kernel_meta(p) for p in programs]
         sources = {}
         for p in programs:
             path = os.path.join(d, f"batch_{p.gid}.hip")
@@ -80,15 +79,19 @@ class BatchPatternSet:
     def n_groups(self):
         return len(self.groups)
 
+    @property
+    def n_kernels(self):
+        return len(self.metas)
+
     def load(self, device=0):
         rt.init(device)
         if self.modules is None:
             self.modules = [rt.Module(h) for h in self.hsacos]
-            self.kernels = [m.kernel(batch_kernel_name(g)) for g, m in enumerate(self.modules)]
+            self.kernels = [m.kernel(batch_kernel_name(k)) for k, m in enumerate(self.modules)]
         return self
 
     def info(self):
-        return [m.info(batch_kernel_name(g)) for g, m in enumerate(self.modules)]
+        return [m.info(batch_kernel_name(k)) for k, m in enumerate(self.modules)]
 
     def free(self):
         if self.modules is not None:
@@ -103,7 +106,7 @@ class BatchPlan:
 
     ``sets``: list of loaded BatchPatternSet; ``buffers``: list of uint8
     arrays; ``jobs``: list of (set index, buffer index) pairs.  Each pair
-    expands to one job per group of the set."""
+    expands to one job per kernel of the set."""
 
     def __init__(self, sets, buffers, jobs, device=0, grid_blocks=None, n_queues=0, take=0):
         import torch
@@ -131,25 +134,28 @@ class BatchPlan:
             o += 8 * W
         self.basis_off = boffs
         self.basis = torch.zeros(max(o, 1), dtype=torch.uint64, device=self.dev)
-        # one work queue per (set, group): the group's jobs over the buffers
+        # one work queue per (set, kernel): the kernel's jobs over the buffers;
+        # a job's result is n_out consecutive streams (its kernel's groups)
         self.jobs = list(jobs)
         res_total = 0
         stat_total = 0
-        self.queues = []   # (set, group) -> queue state
+        self.queues = []   # (set, kernel) -> queue state
         for si, ps in enumerate(sets):
             bufs = [b for (s, b) in self.jobs if s == si]
-            for g, meta in enumerate(ps.metas):
+            for kid, meta in enumerate(ps.metas):
                 C = max(1, meta["n_carry"])
+                n_out = meta.get("n_out", 1)
+                first = meta.get("first_group", kid)
                 entries, tile = [], 0
                 for b in bufs:
                     W = self.words[b]
                     T = n_tiles(meta, W)
-                    entries.append(dict(set=si, group=g, buffer=b, W=W, T=T, C=C, tile_start=tile,
-                                        res_off=res_total, stat_off=stat_total))
-                    res_total += W
+                    entries.append(dict(set=si, kernel=kid, first_group=first, n_out=n_out, buffer=b, W=W, T=T,
+                                        C=C, tile_start=tile, res_off=res_total, stat_off=stat_total))
+                    res_total += W * n_out
                     stat_total += T * C
                     tile += T
-                self.queues.append(dict(set=si, group=g, entries=entries, total_tiles=tile))
+                self.queues.append(dict(set=si, kernel=kid, entries=entries, total_tiles=tile))
         self.result = torch.zeros(max(res_total, 1), dtype=torch.uint64, device=self.dev)
         self.status = torch.zeros(max(stat_total, 1), dtype=torch.int32, device=self.dev)
         for q in self.queues:
@@ -160,7 +166,7 @@ class BatchPlan:
                 arr[k].result = self.result.data_ptr() + 8 * e["res_off"]
                 arr[k].status = self.status.data_ptr() + 4 * e["stat_off"]
                 arr[k].n_words = e["W"]
-                arr[k].group = e["group"]
+                arr[k].group = e["first_group"]
                 arr[k].n_carry = e["C"]
                 arr[k].tile_start = e["tile_start"]
                 arr[k].n_tiles = e["T"]
@@ -210,7 +216,7 @@ class BatchPlan:
         for q in self.queues:
             ps = self.sets[q["set"]]
             if q["total_tiles"]:
-                rt.batch_launch(ps.kernels[q["group"]], q["args"], self._grid(ps, q["total_tiles"]),
+                rt.batch_launch(ps.kernels[q["kernel"]], q["args"], self._grid(ps, q["total_tiles"]),
                                 ps.block_threads, stream)
 
     def enqueue(self, stream=None):
@@ -244,13 +250,14 @@ class BatchPlan:
                 raise rt.HipRuntimeError("carry look-back timed out in a batched launch; results are invalid")
 
     def results(self, set_index, buffer_index):
-        """Match streams [G, W] of one (set, buffer) job (device tensor view)."""
+        """Match streams [G, W] of one (set, buffer) job (device tensor)."""
         rows = [e for q in self.queues if q["set"] == set_index for e in q["entries"] if e["buffer"] == buffer_index]
         if not rows:
             raise KeyError((set_index, buffer_index))
         W = rows[0]["W"]
-        rows.sort(key=lambda e: e["group"])
-        return self.torch.stack([self.result[e["res_off"]:e["res_off"] + W] for e in rows])
+        rows.sort(key=lambda e: e["first_group"])
+        return self.torch.cat([self.result[e["res_off"]:e["res_off"] + W * e["n_out"]].view(e["n_out"], W)
+                               for e in rows])
 
     def scanned_bytes(self):
         return sum(self.lengths[b] for (_, b) in self.jobs)

@@ -83,7 +83,8 @@ def _cname(v):
 class GroupProgram:
     """Analysis + HIP source of one group program."""
 
-    def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False):
+    def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
+                 min_waves=0):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -91,7 +92,9 @@ class GroupProgram:
         common path falls through.  pipeline: compute the first stage of the
         next regex (the part before its first zero test) ahead of the
         current regex's first zero test, so its instructions fill the
-        latencies of the test and of the lane moves."""
+        latencies of the test and of the lane moves.  min_waves: waves per
+        SIMD the register allocator must leave room for (0: its own choice;
+        a higher count may spill)."""
         if lookback not in ("speculate", "wait"):
             raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
         self.gid = gid
@@ -101,6 +104,7 @@ class GroupProgram:
         self.lookback = lookback
         self.expect_zero = bool(expect_zero)
         self.pipeline = bool(pipeline)
+        self.min_waves = int(min_waves)
         self.kernel_name = f"bg_group_{gid}"
         self._check()
         self._halo()
@@ -117,8 +121,11 @@ class GroupProgram:
                 raise HipCodegenError(f"group {self.gid}: unsupported instruction {inst}")
             if t == BsInstType.STREAMSTORE:
                 stores += 1
-        if stores != 1:
-            raise HipCodegenError(f"group {self.gid}: expected exactly one stream store, found {stores}")
+        if stores < 1:
+            raise HipCodegenError(f"group {self.gid}: expected a stream store, found none")
+        # one store per group: a program merged from several groups writes
+        # their streams in order (result + j * W for the j-th store)
+        self.n_out = stores
 
     def _const(self, v):
         n = self.vnm.get(v)
@@ -225,7 +232,12 @@ class GroupProgram:
         for i in range(0, len(vars_), 16):
             chunk = ", ".join(f"{_cname(v)} = 0" for v in vars_[i:i + 16])
             add(f"  uint64_t {chunk};")
-        add("  uint64_t out = 0;")
+        outs = [f"out{j}" for j in range(self.n_out)]
+        add("  uint64_t " + ", ".join(f"{o} = 0" for o in outs) + ";")
+        store_index = {}
+        for inst in self.insts:
+            if inst.type == BsInstType.STREAMSTORE:
+                store_index[id(inst)] = len(store_index)
         carry_index = {id(inst): (k, L) for k, (inst, L) in enumerate(self.carry_ops)}
         # With carry ops the program after its header runs in a loop of
         # passes: a speculative pass and, when bg_settle finds an assumed
@@ -284,7 +296,7 @@ class GroupProgram:
                 zeros = [f"{_cname(v)} = 0;" for v in body_vars]
                 for i in range(0, len(zeros), 16):
                     add("  " + " ".join(zeros[i:i + 16]))
-                add("  out = 0;" + "".join(f" bg_bnd_{k} = -1;" for k in sorted(consumers)))
+                add("  " + " ".join(f"{o} = 0;" for o in outs) + "".join(f" bg_bnd_{k} = -1;" for k in sorted(consumers)))
             t = inst.type
             if t == BsInstType.STREAMLOAD:
                 add(f"  {_cname(inst.ret)} = bas[{int(inst.stream_index)}];")
@@ -323,7 +335,7 @@ class GroupProgram:
                     # recomputed instead of kept live from here
                     add("  " + " ".join(f"bg_fence({_cname(v)});" for v in fence_vars))
             elif t == BsInstType.STREAMSTORE:
-                add(f"  out = {self.val(inst.operands[0])};")
+                add(f"  out{store_index[id(inst)]} = {self.val(inst.operands[0])};")
             elif t == BsInstType.GRAPHBREAK:
                 pass
             else:
@@ -353,7 +365,10 @@ class GroupProgram:
             add("  }")
         add("  const int lo = (t == 0) ? 0 : HL;")
         add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
-        add("  if (inb && lane >= lo && lane < hi) result[w] = out;")
+        add("  if (inb && lane >= lo && lane < hi) {")
+        for j, o in enumerate(outs):
+            add(f"    result[{f'(int64_t){j} * W + ' if j else ''}w] = {o};")
+        add("  }")
         add("}")
         return "\n".join(lines) + "\n"
 
@@ -577,12 +592,17 @@ class GroupProgram:
             merge(cur, res)
         return bounds
 
+    def launch_bounds(self, block_threads=256):
+        if self.min_waves:
+            return f"__launch_bounds__({block_threads}, {self.min_waves})"
+        return f"__launch_bounds__({block_threads})"
+
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
         return "\n".join([
             '#include "bitgen_device.h"',
             self.device_function(),
-            f'extern "C" __global__ __launch_bounds__(256) void {self.kernel_name}(bg_group_args_t a) {{',
+            f'extern "C" __global__ {self.launch_bounds()} void {self.kernel_name}(bg_group_args_t a) {{',
             "  // persistent: each wave takes tiles from its block's queue until the",
             "  // queue is exhausted",
             "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles, a.take);",
@@ -608,6 +628,7 @@ class GroupProgram:
             "HR": self.HR,
             "S": self.S,
             "n_carry": self.n_carry,
+            "n_out": self.n_out,
             "n_insts": len(self.insts),
             "regexes": list(self.regexes),
         }
@@ -626,7 +647,7 @@ def batch_source(program, block_threads=256):
     return "\n".join([
         '#include "bitgen_device.h"',
         program.device_function(),
-        f'extern "C" __global__ __launch_bounds__({block_threads}) void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
+        f'extern "C" __global__ {program.launch_bounds(block_threads)} void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
         "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles, a.take);",
         "  uint32_t first;",
         "  bool more = bg_queue_take(Q, first);",

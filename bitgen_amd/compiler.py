@@ -39,6 +39,11 @@ DEFAULT_OPTIONS = {
     # first stage computed ahead of the previous regex's first zero test
     "sched_expect": 1,
     "sched_pipeline": 0,
+    # waves per SIMD the register allocator must leave room for (0: its own)
+    "min_waves": 0,
+    # groups merged into one kernel (one launch computes all their streams
+    # and shares their basis loads, class header, tile queue and look-back)
+    "groups_per_kernel": 1,
 }
 
 
@@ -105,10 +110,95 @@ def group_programs(regexes, options=None):
     return pre, pre_vnm, post, post_vnm
 
 
+_PURE = None
+
+
+def merge_programs(parts):
+    """One program that computes several groups' match streams.
+
+    ``parts``: [(post-pass insts, vnm)] of the groups, in order.  Variable ids
+    and skip labels of part j are moved past those of the parts before it;
+    the basis loads of every part become those of part 0; and a pure,
+    single-definition instruction before its part's first zero test that
+    computes the same function of the same inputs as an earlier one is
+    dropped, its result renamed to the earlier one (the groups share their
+    character-class header).  Each part keeps its STREAMSTORE, in part
+    order: store j writes group j's stream.  Returns (insts, vnm)."""
+    from .inst import BsInstType, walk
+
+    global _PURE
+    if _PURE is None:
+        _PURE = {BsInstType.AND, BsInstType.OR, BsInstType.XOR, BsInstType.NOT, BsInstType.ADVANCE,
+                 BsInstType.ASSIGN}
+    commutative = {BsInstType.AND, BsInstType.OR, BsInstType.XOR}
+    merged, vnm = [], {}
+    next_id, next_label = 0, 0
+    known = {}   # expression key -> canonical variable id
+    canon = {}   # renamed id -> canonical id
+    for insts, pvnm in parts:
+        ids = set(pvnm)
+        labels = set()
+        ndefs = {}
+        for inst in walk(insts):
+            ids.update(inst.defs())
+            ids.update(inst.uses())
+            for d in inst.defs():
+                ndefs[d] = ndefs.get(d, 0) + 1
+            if inst.type in (BsInstType.SKIPZERO, BsInstType.LABEL):
+                labels.add(inst.label)
+        base = next_id
+        next_id = base + (max(ids) + 1 if ids else 0)
+        lbase = next_label
+        next_label = lbase + (max(labels) + 1 if labels else 0)
+        for v, name in pvnm.items():
+            vnm[base + v] = name
+        first_skip = next((k for k, i in enumerate(insts) if i.type == BsInstType.SKIPZERO), len(insts))
+
+        def operand_key(v):
+            name = vnm.get(v)
+            return ("c", name) if name in ("0", "1") else ("v", canon.get(v, v))
+
+        def moved(inst):
+            """Copy of inst (and of any nested loop or branch body) with this
+            part's ids, labels and the shared-header renames applied."""
+            new = inst.deepcopy()
+            new.replace_uses({u: canon.get(base + u, base + u) for u in inst.uses()})
+            if new.ret is not None:
+                new.ret = base + new.ret
+            if new.type in (BsInstType.SKIPZERO, BsInstType.LABEL):
+                new.label = lbase + new.label
+            if new.type == BsInstType.WHILE:
+                new.body = [moved(i) for i in inst.body]
+            elif new.type == BsInstType.IF:
+                new.body_1 = [moved(i) for i in inst.body_1]
+                new.body_2 = [moved(i) for i in inst.body_2] if inst.body_2 else inst.body_2
+            return new
+
+        for k, inst in enumerate(insts):
+            new = moved(inst)
+            key = None
+            if new.type == BsInstType.STREAMLOAD:
+                key = ("load", int(new.stream_index))
+            elif new.type in _PURE and k < first_skip and ndefs.get(inst.ret) == 1:
+                ops = [operand_key(u) for u in new.uses()]
+                if new.type in commutative:
+                    ops.sort()
+                extra = (new.amount, new.is_left) if new.type == BsInstType.ADVANCE else ()
+                key = (new.type.value, tuple(ops), extra)
+            if key is not None:
+                if key in known:
+                    canon[new.ret] = known[key]
+                    continue
+                known[key] = new.ret
+            merged.append(new)
+    return merged, vnm
+
+
 def program_for(gid, insts, vnm, regexes, options):
     """GroupProgram of one group's post-pass IR under the compile options."""
     return GroupProgram(gid, insts, vnm, regexes, lookback=options["lookback"],
-                        expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]))
+                        expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]),
+                        min_waves=int(options["min_waves"]))
 
 
 @dataclass
@@ -141,8 +231,10 @@ def code_digest():
         for sub in ("tool", "bitparser", "inst", "passes", "codegen"):
             d = os.path.join(_PKG, sub)
             paths += [os.path.join(d, f) for f in sorted(os.listdir(d)) if f.endswith(".py")]
+        root = os.path.dirname(_PKG)
         for path in paths:
-            h.update(path.encode())
+            # relative names: the digest must not depend on where the tree lives
+            h.update(os.path.relpath(path, root).encode())
             with open(path, "rb") as f:
                 h.update(f.read())
         _code_digest = h.hexdigest()[:24]
@@ -157,12 +249,21 @@ def _set_dir(groups, options):
     return d
 
 
+def _index_path(d, name):
+    """Cached metas of a set live apart from its generated sources (a small
+    index file per set and form), so the sources can be dropped without
+    losing the cache."""
+    root = cfg.get_config("cache_dir", ".cache_bitgen")
+    os.makedirs(os.path.join(root, "index"), exist_ok=True)
+    return os.path.join(root, "index", f"{os.path.basename(os.path.normpath(d))}-{name}")
+
+
 def load_cached_set(d, name):
     """Metas of a set compiled earlier under the same code digest whose code
     objects are all still present, else None."""
     from .codegen.gen_hip import hsaco_cache_dir
 
-    path = os.path.join(d, name)
+    path = _index_path(d, name)
     try:
         with open(path) as f:
             rec = json.load(f)
@@ -181,14 +282,44 @@ def load_cached_set(d, name):
 
 def store_cached_set(d, name, metas):
     rec = {"digest": code_digest(), "metas": [dict(m, hsaco=os.path.basename(m["hsaco"])) for m in metas]}
-    tmp = os.path.join(d, name + ".tmp")
-    with open(tmp, "w") as f:
+    path = _index_path(d, name)
+    with open(path + ".tmp", "w") as f:
         json.dump(rec, f)
-    os.replace(tmp, os.path.join(d, name))
+    os.replace(path + ".tmp", path)
+
+
+def kernel_groups(n_groups, groups_per_kernel):
+    """Groups of each kernel: consecutive runs of ``groups_per_kernel``."""
+    k = max(1, int(groups_per_kernel))
+    return [list(range(i, min(i + k, n_groups))) for i in range(0, n_groups, k)]
+
+
+def kernel_programs(groups, o):
+    """One GroupProgram per kernel: the kernel's groups merged into one
+    program (their streams are the program's stores, in group order)."""
+    programs = []
+    for kid, members in enumerate(kernel_groups(len(groups), o["groups_per_kernel"])):
+        parts = []
+        for g in members:
+            _, _, post, post_vnm = group_programs(groups[g], o)
+            parts.append((post, post_vnm))
+        insts, vnm = parts[0] if len(parts) == 1 else merge_programs(parts)
+        prog = program_for(kid, insts, vnm, [r for g in members for r in groups[g]], o)
+        prog.groups = members
+        programs.append(prog)
+    return programs
+
+
+def kernel_meta(prog):
+    m = prog.meta()
+    m["groups"] = list(prog.groups)
+    m["first_group"] = prog.groups[0]
+    return m
 
 
 def compile_pattern_set(groups, options=None, workers=None):
-    """Compile regex groups (list of lists) into gfx950 code objects."""
+    """Compile regex groups (list of lists) into gfx950 code objects: one
+    kernel per ``groups_per_kernel`` consecutive groups."""
     o = dict(DEFAULT_OPTIONS)
     o.update(options or {})
     groups = [list(g) for g in groups]
@@ -196,12 +327,10 @@ def compile_pattern_set(groups, options=None, workers=None):
     cached = load_cached_set(d, "plain.json")
     if cached is not None:
         return CompiledPatternSet(groups, cached, d, o)
-    programs, sources = [], {}
-    for gid, regs in enumerate(groups):
-        _, _, post, post_vnm = group_programs(regs, o)
-        prog = program_for(gid, post, post_vnm, regs, o)
-        programs.append(prog)
-        path = os.path.join(d, f"kernel_{gid}.hip")
+    programs = kernel_programs(groups, o)
+    sources = {}
+    for prog in programs:
+        path = os.path.join(d, f"kernel_{prog.gid}.hip")
         src = prog.source()
         with open(path, "w") as f:
             f.write(src)
@@ -209,7 +338,7 @@ def compile_pattern_set(groups, options=None, workers=None):
     hsacos = compile_sources(sources, workers=workers)
     metas = []
     for prog in programs:
-        m = prog.meta()
+        m = kernel_meta(prog)
         m["hsaco"] = os.path.abspath(hsacos[os.path.join(d, f"kernel_{prog.gid}.hip")])
         metas.append(m)
     store_cached_set(d, "plain.json", metas)

@@ -4,7 +4,7 @@ Jobs (pattern-set id + byte buffer) arrive at any time through ``submit``;
 a scheduler thread keeps the pattern sets' batch kernels loaded on the GPU,
 admits waiting jobs while their HBM footprint fits under a cap, packs the
 admitted jobs into one batch (a batched transpose plus one persistent launch
-per group of every set involved) on one of several streams, and completes
+per kernel of every set involved) on one of several streams, and completes
 jobs when their batch's event fires.  Jobs that arrive while a batch is
 running are admitted into the next batch on another stream, so their work
 overlaps on the device.  Each job's result is its own buffer's match streams,
@@ -65,15 +65,47 @@ def admit(waiting, in_flight_bytes, cap, max_jobs, max_bytes):
 
 
 class _Slot:
-    """One stream with its in-flight batch."""
+    """One stream with its in-flight batch and its own buffers, reused from
+    batch to batch and grown (never shrunk) when a batch needs more: pinned
+    host staging for the input bytes and the launch tables, device buffers
+    for input, basis, match streams, look-back status and tables, a pinned
+    host buffer for the match streams, and one control block per (set,
+    kernel) with a pinned copy of their error flags.  A slot's buffers are
+    touched only while it holds no batch, so reuse never races the GPU."""
 
-    def __init__(self, torch, dev):
+    # initial capacities (elements): enough for a batch of a few MiB, so
+    # steady serving never allocates (pinned allocations take milliseconds)
+    INITIAL = {"dev_in": 8 << 20, "basis": 1 << 20, "result": 4 << 20, "status": 1 << 20,
+               "host_in": 8 << 20, "host_meta": 1 << 20, "dev_meta": 1 << 20, "host_out": 4 << 20}
+
+    def __init__(self, torch, dev, n_ctl):
+        self.torch = torch
+        self.dev = dev
         self.stream = torch.cuda.Stream(device=dev)
+        self.ctl = torch.zeros((max(1, n_ctl), rt.CTL_WORDS), dtype=torch.int32, device=dev)
+        self.err_host = torch.zeros(max(1, n_ctl), dtype=torch.int32, pin_memory=True)
+        self.bufs = {}
         self.event = None
         self.jobs = None
-        self.keep = None
-        self.host_out = None
         self.out_rows = None
+        self.t_launch = 0.0
+        dtypes = {"dev_in": torch.uint8, "basis": torch.uint64, "result": torch.uint64, "status": torch.int32,
+                  "host_in": torch.uint8, "host_meta": torch.uint8, "dev_meta": torch.uint8,
+                  "host_out": torch.uint64}
+        for name, n in self.INITIAL.items():
+            self.buf(name, n, dtypes[name], pinned=name.startswith("host"))
+
+    def buf(self, name, n, dtype, pinned=False):
+        """A buffer of at least n elements (contents undefined)."""
+        t = self.bufs.get(name)
+        if t is None or t.numel() < n:
+            size = max(int(n), int(1.5 * t.numel()) if t is not None else 0, 1024)
+            if pinned:
+                t = self.torch.empty(size, dtype=dtype, pin_memory=True)
+            else:
+                t = self.torch.empty(size, dtype=dtype, device=self.dev)
+            self.bufs[name] = t
+        return t
 
 
 class MatchServer:
@@ -92,14 +124,20 @@ class MatchServer:
         self.inbox = queue.Queue()
         self.waiting = []
         self.in_flight_bytes = 0
-        self.slots = [_Slot(torch, self.dev) for _ in range(n_streams)]
+        # control-block row of every (set, kernel) pair, the same in each slot
+        self.ctl_row = {}
+        for sid, ps in self.sets.items():
+            for kid in range(len(ps.metas)):
+                self.ctl_row[(sid, kid)] = len(self.ctl_row)
+        self.slots = [_Slot(torch, self.dev, len(self.ctl_row)) for _ in range(n_streams)]
         self.completed = []
         self.batches = 0
+        # per completed batch: (jobs, host staging seconds, launch-to-done seconds)
+        self.batch_log = []
         self._stop = threading.Event()
         self._thread = None
         props = torch.cuda.get_device_properties(self.dev)
         self.grid_waves = props.multi_processor_count * grid_waves_per_cu
-        self.ctl = {}  # (set, group, slot) -> control block (tile queues, error flag, epoch)
 
     # -- client side -------------------------------------------------------------
     def submit(self, set_id, data):
@@ -122,7 +160,7 @@ class MatchServer:
 
     def stop(self, drain=True):
         if drain:
-            while not self.inbox.empty() or self.waiting or any(s.jobs for s in self.slots):
+            while not self.inbox.empty() or self.waiting or any(s.jobs is not None for s in self.slots):
                 time.sleep(0.001)
         self._stop.set()
         if self._thread:
@@ -146,7 +184,7 @@ class MatchServer:
                     self._launch(free[0], batch)
                     progressed = True
             if not progressed:
-                if self.waiting or any(s.jobs for s in self.slots):
+                if self.waiting or any(s.jobs is not None for s in self.slots):
                     time.sleep(20e-6)
                 else:
                     try:
@@ -160,84 +198,106 @@ class MatchServer:
         return batch
 
     def _launch(self, slot, jobs):
+        """Stage a batch on a free slot: one copy of all input bytes and one
+        of all launch tables, one batched transpose, one persistent launch
+        per (set, kernel) involved, one copy of all match streams back."""
         torch = self.torch
+        slot.t_start = time.perf_counter()
         n = len(jobs)
         lengths = np.array([j.data.size for j in jobs], dtype=np.int64)
         words = (lengths + 1 + 63) // 64
-        byte_off = np.concatenate([[0], np.cumsum((lengths + 63) // 64 * 64)])[:-1]
-        total_bytes = int(((lengths + 63) // 64 * 64).sum()) or 64
-        host = torch.empty(total_bytes, dtype=torch.uint8, pin_memory=True)
-        hv = host.numpy()
+        padded = (lengths + 63) // 64 * 64
+        byte_off = np.concatenate([[0], np.cumsum(padded)])[:-1]
+        total_bytes = int(padded.sum()) or 64
+        basis_off = np.concatenate([[0], np.cumsum(8 * words)])[:-1]
+        # per (set, kernel) queues over this batch's jobs of that set; a job's
+        # result holds its kernel's n_out group streams in order
+        queues, res_total, stat_total = [], 0, 0
+        rows = []  # (job index, group, res_off, W)
+        by_set = {}
+        for i, j in enumerate(jobs):
+            by_set.setdefault(j.set_id, []).append(i)
+        for sid, idx in by_set.items():
+            ps = self.sets[sid]
+            idx = np.array(idx, dtype=np.int64)
+            for kid, meta in enumerate(ps.metas):
+                C = max(1, meta["n_carry"])
+                n_out = meta.get("n_out", 1)
+                first = meta.get("first_group", kid)
+                T = np.array([n_tiles(meta, int(words[i])) for i in idx], dtype=np.int64)
+                ent = np.zeros(len(idx), dtype=_JOB_DTYPE)
+                ent["n_words"] = words[idx]
+                ent["group"] = first
+                ent["n_carry"] = C
+                ent["tile_start"] = np.concatenate([[0], np.cumsum(T)])[:-1]
+                ent["n_tiles"] = T
+                res_off = res_total + np.concatenate([[0], np.cumsum(words[idx] * n_out)])[:-1]
+                stat_off = stat_total + np.concatenate([[0], np.cumsum(T * C)])[:-1]
+                res_total += int(words[idx].sum()) * n_out
+                stat_total += int((T * C).sum())
+                queues.append((sid, kid, idx, ent, res_off, stat_off, int(T.sum()), T))
+                rows.extend((int(i), first + j, int(r) + j * int(words[i]), int(words[i]))
+                            for i, r in zip(idx, res_off) for j in range(n_out))
+        res_total, stat_total = max(res_total, 1), max(stat_total, 1)
+        # device buffers of this slot (grown if needed), then the tables that
+        # point into them, packed into one byte image with 256-byte sections
+        dbytes = slot.buf("dev_in", total_bytes, torch.uint8)
+        basis = slot.buf("basis", int(8 * words.sum()) or 1, torch.uint64)
+        result = slot.buf("result", res_total, torch.uint64)
+        status = slot.buf("status", stat_total, torch.int32)
+        td = np.zeros(n, dtype=_TDESC_DTYPE)
+        td["byte_off"] = byte_off
+        td["n_bytes"] = lengths
+        td["basis_off"] = basis_off
+        td["n_words"] = words
+        td["word_start"] = np.concatenate([[0], np.cumsum(words)])[:-1]
+        sections = [td.view(np.uint8)]
+        for (sid, kid, idx, ent, res_off, stat_off, total, T) in queues:
+            ent["basis"] = basis.data_ptr() + 8 * basis_off[idx]
+            ent["result"] = result.data_ptr() + 8 * res_off
+            ent["status"] = status.data_ptr() + 4 * stat_off
+            sections.append(ent.view(np.uint8))
+            sections.append(np.repeat(np.arange(len(idx), dtype=np.int32), T).view(np.uint8))
+        offs, o = [], 0
+        for sec in sections:
+            offs.append(o)
+            o += (sec.size + 255) // 256 * 256
+        meta_bytes = max(o, 256)
+        host_in = slot.buf("host_in", total_bytes, torch.uint8, pinned=True)
+        hv = host_in.numpy()
         for j, off in zip(jobs, byte_off):
             hv[off:off + j.data.size] = j.data
-        basis_off = np.concatenate([[0], np.cumsum(8 * words)])[:-1]
+        host_meta = slot.buf("host_meta", meta_bytes, torch.uint8, pinned=True)
+        mv = host_meta.numpy()
+        for sec, off in zip(sections, offs):
+            mv[off:off + sec.size] = sec
+        dmeta = slot.buf("dev_meta", meta_bytes, torch.uint8)
+        host_out = slot.buf("host_out", res_total, torch.uint64, pinned=True)
         with torch.cuda.stream(slot.stream):
-            dbytes = torch.empty(total_bytes, dtype=torch.uint8, device=self.dev)
-            dbytes.copy_(host, non_blocking=True)
-            basis = torch.empty(int(8 * words.sum()), dtype=torch.uint64, device=self.dev)
-            # per (set, group) queues over this batch's jobs of that set
-            queues, res_total, stat_total = [], 0, 0
-            rows = []  # (job index, group, res_off, W)
-            by_set = {}
-            for i, j in enumerate(jobs):
-                by_set.setdefault(j.set_id, []).append(i)
-            for sid, idx in by_set.items():
-                ps = self.sets[sid]
-                for g, meta in enumerate(ps.metas):
-                    C = max(1, meta["n_carry"])
-                    T = np.array([n_tiles(meta, int(words[i])) for i in idx], dtype=np.int64)
-                    starts = np.concatenate([[0], np.cumsum(T)])[:-1]
-                    ent = np.zeros(len(idx), dtype=_JOB_DTYPE)
-                    ent["n_words"] = words[idx]
-                    ent["group"] = g
-                    ent["n_carry"] = C
-                    ent["tile_start"] = starts
-                    ent["n_tiles"] = T
-                    res_off = res_total + np.concatenate([[0], np.cumsum(words[idx])])[:-1]
-                    stat_off = stat_total + np.concatenate([[0], np.cumsum(T * C)])[:-1]
-                    res_total += int(words[idx].sum())
-                    stat_total += int((T * C).sum())
-                    queues.append((sid, g, idx, ent, res_off, stat_off, int(T.sum()), T))
-                    rows.extend((i, g, int(r), int(words[i])) for i, r in zip(idx, res_off))
-            result = torch.empty(max(res_total, 1), dtype=torch.uint64, device=self.dev)
-            status = torch.zeros(max(stat_total, 1), dtype=torch.int32, device=self.dev)
-            td = np.zeros(n, dtype=_TDESC_DTYPE)
-            td["byte_off"] = byte_off
-            td["n_bytes"] = lengths
-            td["basis_off"] = basis_off
-            td["n_words"] = words
-            td["word_start"] = np.concatenate([[0], np.cumsum(words)])[:-1]
-            tdesc = torch.from_numpy(td.view(np.uint8)).pin_memory().to(self.dev, non_blocking=True)
+            dbytes[:total_bytes].copy_(host_in[:total_bytes], non_blocking=True)
+            dmeta[:meta_bytes].copy_(host_meta[:meta_bytes], non_blocking=True)
+            # status words must start clear: regions move between (set,
+            # kernel) pairs from batch to batch, and their epochs differ
+            status[:stat_total].zero_()
             s = rt.current_stream_ptr(slot.stream)
-            rt.transpose_batch(dbytes.data_ptr(), basis.data_ptr(), tdesc.data_ptr(), n, int(words.sum()), s)
-            keep = [host, dbytes, basis, result, status, tdesc]
-            for (sid, g, idx, ent, res_off, stat_off, total, T) in queues:
+            rt.transpose_batch(dbytes.data_ptr(), basis.data_ptr(), dmeta.data_ptr() + offs[0], n, int(words.sum()), s)
+            for q, (sid, kid, idx, ent, res_off, stat_off, total, T) in enumerate(queues):
                 ps = self.sets[sid]
-                ent["basis"] = basis.data_ptr() + 8 * basis_off[idx]
-                ent["result"] = result.data_ptr() + 8 * res_off
-                ent["status"] = status.data_ptr() + 4 * stat_off
-                jt = torch.from_numpy(ent.view(np.uint8)).pin_memory().to(self.dev, non_blocking=True)
-                tj_host = np.repeat(np.arange(len(idx), dtype=np.int32), T)
-                tj = torch.from_numpy(tj_host).pin_memory().to(self.dev, non_blocking=True)
-                key = (sid, g, id(slot))
-                if key not in self.ctl:
-                    self.ctl[key] = rt.new_ctl(self.dev)
-                ctl = self.ctl[key]
                 args = rt.BatchArgs()
-                args.jobs = jt.data_ptr()
-                args.tile_job = tj.data_ptr()
-                args.ctl = ctl.data_ptr()
+                args.jobs = dmeta.data_ptr() + offs[1 + 2 * q]
+                args.tile_job = dmeta.data_ptr() + offs[2 + 2 * q]
+                args.ctl = slot.ctl[self.ctl_row[(sid, kid)]].data_ptr()
                 args.n_jobs = len(idx)
                 args.total_tiles = total
                 wpb = ps.block_threads // 64
                 blocks = max(1, min(self.grid_waves // wpb, (total + wpb - 1) // wpb))
-                rt.batch_launch(ps.kernels[g], args, blocks, ps.block_threads, s)
-                keep += [jt, tj]
-            host_out = torch.empty(max(res_total, 1), dtype=torch.uint64, pin_memory=True)
-            host_out.copy_(result, non_blocking=True)
+                rt.batch_launch(ps.kernels[kid], args, blocks, ps.block_threads, s)
+            host_out[:res_total].copy_(result[:res_total], non_blocking=True)
+            slot.err_host.copy_(slot.ctl[:, rt.CTL_ERROR], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
-        slot.event, slot.jobs, slot.keep, slot.host_out, slot.out_rows = ev, jobs, keep, host_out, rows
+        slot.event, slot.jobs, slot.out_rows = ev, jobs, rows
+        slot.t_launch = time.perf_counter()
         self.batches += 1
 
     def _complete(self):
@@ -245,11 +305,14 @@ class MatchServer:
         for slot in self.slots:
             if slot.jobs is None or not slot.event.query():
                 continue
-            out = slot.host_out.numpy()
+            if int(slot.err_host.abs().sum()) != 0:
+                raise rt.HipRuntimeError("carry look-back timed out in a served batch")
+            out = slot.bufs["host_out"].numpy()
             per_job = {}
             for (i, g, off, W) in slot.out_rows:
                 per_job.setdefault(i, {})[g] = out[off:off + W]
             now = time.perf_counter()
+            self.batch_log.append((len(slot.jobs), slot.t_launch - slot.t_start, now - slot.t_launch))
             for i, job in enumerate(slot.jobs):
                 gs = per_job.get(i, {})
                 res = np.stack([gs[g] for g in sorted(gs)]).copy() if gs else np.zeros((0, 0), np.uint64)
@@ -257,10 +320,7 @@ class MatchServer:
                 self.in_flight_bytes -= job.footprint
                 self.completed.append(job)
                 job.future.set_result(res)
-            for (sid, g, sl), ctl in self.ctl.items():
-                if sl == id(slot) and int(ctl[rt.CTL_ERROR].item()) != 0:
-                    raise rt.HipRuntimeError("carry look-back timed out in a served batch")
-            slot.jobs = slot.keep = slot.host_out = slot.out_rows = slot.event = None
+            slot.jobs = slot.out_rows = slot.event = None
             progressed = True
         return progressed
 

@@ -220,7 +220,9 @@ __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, u
   uint32_t spins = 0;
   bool have = st != 0;
   while (true) {
-    if (!have) st = bg_st_load(tc.status + (size_t)j * tc.n_carry + K);
+    // every lane loads the same word; readfirstlane keeps the value (and
+    // everything decided from it) in scalar registers
+    if (!have) st = (uint32_t)__builtin_amdgcn_readfirstlane((int)bg_st_load(tc.status + (size_t)j * tc.n_carry + K));
     have = false;
     const uint32_t kind = (st >> 2) & 3u;
     if ((st >> 4) != (tc.epoch & 0x0FFFFFFFu) || kind == 0) {
@@ -334,7 +336,10 @@ __device__ __forceinline__ uint64_t bg_wave_add(uint64_t a, uint64_t b, bg_tile 
     hi = hi > 63 ? 63 : hi;
     infl = hi > b ? hi : b;
   }
+  // cm is wave-uniform: most tiles generate no carry between lanes at all,
+  // and then the per-lane carry add is skipped
   const uint64_t cm = (GP + G + ((uint64_t)cin << l0)) ^ P;
+  if (cm == 0) return s;
   const uint64_t carry = (cm >> lane) & 1ull;
   return s + carry;
 }

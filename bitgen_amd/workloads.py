@@ -176,12 +176,20 @@ def text_ids(n, seed, printable=0.7):
 
 
 def text_dna(n, seed, n_run_rate=0.001, line=80):
+    """i.i.d. uniform bases, runs of N (length 10-100, starting at about
+    n_run_rate/55 of the positions), LF every ``line`` bytes.  Generated in
+    64 MiB chunks of uint8 draws so 1 GiB needs no larger temporaries."""
     g = np.random.default_rng(seed)
-    buf = np.frombuffer(b"ACGT", dtype=np.uint8)[g.integers(0, 4, size=n)]
-    buf = buf.copy()
-    starts = np.nonzero(g.random(n) < n_run_rate / 55.0)[0]
-    for s in starts:
-        ln = int(g.integers(10, 101))
+    lut = np.frombuffer(b"ACGT", dtype=np.uint8)
+    buf = np.empty(n, dtype=np.uint8)
+    step = 1 << 26
+    for o in range(0, n, step):
+        m = min(step, n - o)
+        buf[o:o + m] = lut[g.integers(0, 4, size=m, dtype=np.uint8)]
+    k = int(g.binomial(n, n_run_rate / 55.0)) if n else 0
+    starts = np.sort(g.integers(0, max(n, 1), size=k))
+    lens = g.integers(10, 101, size=k)
+    for s, ln in zip(starts, lens):
         buf[s:s + ln] = ord("N")
     buf[line - 1::line] = LF
     return buf

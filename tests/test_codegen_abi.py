@@ -80,7 +80,7 @@ def test_pass_loop_replays_exactly(regexes):
     inside = {d for i in p.insts[p.loop_start:] for d in i.defs()}
     assert not (before & inside)
     loop = src[src.index("for (;;) {"):]
-    resets = loop[:loop.index("out = 0;")]
+    resets = loop[:loop.index("out0 = 0;")]
     from bitgen_amd.codegen.gen_hip import _cname
     for v in inside:
         assert f"{_cname(v)} = 0;" in resets

@@ -70,3 +70,19 @@ def test_batched_partial_job_list(gpu):
     plan.enqueue()
     _check(plan, sets, buffers, jobs)
     plan.destroy()
+
+
+def test_batched_merged_kernels(gpu):
+    """Batch kernels that compute two or three groups each: per-group
+    streams of every job stay bit-exact."""
+    sets = [BatchPatternSet(s, {"groups_per_kernel": k}).load(0) for s, k in zip(SETS, (2, 3))]
+    assert [s.n_kernels for s in sets] == [1, 1]
+    buffers = _buffers(3, 10)
+    jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers))]
+    print(f"buffers={[b.size for b in buffers]}", flush=True)
+    plan = BatchPlan(sets, buffers, jobs)
+    plan.enqueue()
+    _check(plan, sets, buffers, jobs)
+    plan.destroy()
+    for s in sets:
+        s.free()

@@ -44,6 +44,20 @@ def test_random_groups(gpu, seed):
     assert_parity(groups, data, gpu, repeat=2, label=f"seed {seed}")
 
 
+@pytest.mark.parametrize("seed,per_kernel", [(s, k) for s in range(6) for k in (2, 3)])
+def test_merged_group_kernels(gpu, seed, per_kernel):
+    """Several groups merged into one kernel: every group's stream is still
+    its own, bit for bit, at every tile of long and short inputs."""
+    rng = random.Random(50 + seed)
+    alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
+    n = rng.choice([50, 4000, 70000, 300001])
+    data = np.random.default_rng(50 + seed).choice(alpha, size=n)
+    groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(2, 5))]
+    print(f"seed={seed} per_kernel={per_kernel} n={n} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, options={"groups_per_kernel": per_kernel}, repeat=2,
+                  label=f"merged seed {seed}")
+
+
 def test_plain_small_workload(gpu):
     regexes, n_groups, buf = workloads.plain_small()
     from bitgen_amd.compiler import group_regex_in_list

@@ -132,3 +132,35 @@ def test_group_balancing_matches_reference_rule():
     # character: a 1, \\x00+ 2, abc 3, xyz 3, [a-z]{3} 4, abcdef 6 -- then
     # each to the currently lightest group (scripts/run_bitgen.py:102-116)
     assert groups == [["a", "abc", "[a-z]{3}"], ["\\x00+", "xyz", "abcdef"]]
+
+
+@pytest.mark.parametrize("seed", range(20))
+def test_merged_program_computes_every_group(seed):
+    """Several groups merged into one program (one kernel): each store of
+    the merged program gives exactly its group's own stream, and the groups
+    share their basis loads and class header."""
+    from bitgen_amd.codegen.gen_hip import GroupProgram
+
+    rng = random.Random(100 + seed)
+    groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(2, 4))]
+    n = rng.choice([0, 50, 900, 5000])
+    data = np.frombuffer(bytes(rng.choice(b"abcdxyzq0123\n") for _ in range(n)), dtype=np.uint8)
+    parts = [compiler.group_programs(g)[2:] for g in groups]
+    merged, vnm = compiler.merge_programs(parts)
+    stores = [i for i, inst in enumerate(merged) if inst.type == BsInstType.STREAMSTORE]
+    assert len(stores) == len(groups)
+    assert sum(inst.type == BsInstType.STREAMLOAD for inst in merged) <= 8
+    assert len(merged) < sum(len(p[0]) for p in parts)
+    labels = [inst.label for inst in merged if inst.type == BsInstType.LABEL]
+    assert len(labels) == len(set(labels))
+    basis = O.transpose(data)
+    for j in range(len(groups)):
+        keep = [inst for i, inst in enumerate(merged) if inst.type != BsInstType.STREAMSTORE or i == stores[j]]
+        got = O.mask_to_positions(O.run(keep, vnm, basis), n + 1)
+        ref = O.mask_to_positions(O.run(parts[j][0], parts[j][1], basis), n + 1)
+        assert np.array_equal(got, ref), (groups, j)
+    if any(inst.type == BsInstType.WHILE for inst in walk(merged)):
+        return  # loops other than stars have no hip form
+    src = GroupProgram(0, merged, vnm).source()
+    last = len(groups) - 1
+    assert f"result[(int64_t){last} * W + w] = out{last};" in src
