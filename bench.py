@@ -253,12 +253,14 @@ def served_measurement(args):
     for rate in [float(r) for r in args.served_rates.split(",") if r]:
         arrivals, set_idx, bufs = workloads.serving(args.served_jobs, rate, seed=11 + int(rate))
         nbytes = int(sum(b.size for b in bufs))
-        # warm the server's paths once
-        warm = MatchServer(bsets).start()
-        warm.submit(0, bufs[0]).result(timeout=60)
-        warm.stop()
-
         srv = MatchServer(bsets, hbm_cap_bytes=200 * 10**9).start()
+        # first launches of every set's kernels (and the server's paths) are
+        # warmed before the clock starts
+        for f in [srv.submit(i, bufs[0]) for i in bsets]:
+            f.result(timeout=60)
+        srv.completed.clear()
+        srv.batch_log.clear()
+        srv.batches = 0
         futs = [None] * len(bufs)
 
         def producer():
@@ -282,7 +284,8 @@ def served_measurement(args):
 
         # per-job plain launches, FIFO on one host thread, same arrivals
         lat_plain = np.empty(len(bufs))
-        run_plain_job(psets[0], bufs[0], dev)
+        for i in psets:
+            run_plain_job(psets[i], bufs[0], dev)
         torch.cuda.synchronize()
         t_start = time.perf_counter()
         for i, (a, s, b) in enumerate(zip(arrivals, set_idx, bufs)):
@@ -299,6 +302,7 @@ def served_measurement(args):
             "bytes": nbytes,
             "served_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
             "served_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "served_max_ms": round(float(lat.max()) * 1e3, 3),
             "served_gbs": round(served_gbs, 4),
             "served_batches": srv.batches,
             "served_jobs_per_batch_mean": round(float(blog[:, 0].mean()), 2),
@@ -317,19 +321,24 @@ def served_measurement(args):
     return out
 
 
-def profiled_traffic(args):
+def profiled_traffic(args, options):
     """HBM bytes per group-kernel launch measured by rocprofv3 (FETCH_SIZE and
     WRITE_SIZE, scaled by a plain copy kernel) for the default workload, from
-    the committed counter summary; None for other workloads."""
+    the committed counter summary; None for other workloads or another launch
+    shape than the one profiled."""
     if (args.mib, args.regexes, args.groups) != (100, 1000, 16):
         return None
     path = os.path.join(ROOT, "profiles", "bg_group_ids_100MiB_counters.md")
     if not os.path.exists(path):
         return None
+    traffic, shape = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch"):
-            return int(float(line.split("=")[1]))
-    return None
+            traffic = int(float(line.split("=")[1]))
+        elif line.startswith("shape ="):
+            shape = dict(kv.split("=") for kv in line.split("=", 1)[1].split())
+    want = {"groups_per_kernel": str(options.get("groups_per_kernel", 1)), "min_waves": str(options.get("min_waves", 0))}
+    return traffic if shape == want else None
 
 
 def main():
@@ -477,7 +486,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": profiled_traffic(args),
+            "traffic": profiled_traffic(args, cs.options),
         },
     }
     if count is not None:

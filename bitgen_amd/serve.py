@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .backend.hip import runtime as rt
-from .batch import n_tiles, words_for
+from .batch import words_for
 
 _JOB_DTYPE = np.dtype([("basis", "<u8"), ("result", "<u8"), ("status", "<u8"), ("n_words", "<u8"),
                        ("group", "<u4"), ("n_carry", "<u4"), ("tile_start", "<u4"), ("n_tiles", "<u4")])
@@ -211,33 +211,31 @@ class MatchServer:
         total_bytes = int(padded.sum()) or 64
         basis_off = np.concatenate([[0], np.cumsum(8 * words)])[:-1]
         # per (set, kernel) queues over this batch's jobs of that set; a job's
-        # result holds its kernel's n_out group streams in order
+        # result holds its kernel's n_out group streams in order.  Everything
+        # per job is vectorised: a batch can hold hundreds of jobs.
         queues, res_total, stat_total = [], 0, 0
-        rows = []  # (job index, group, res_off, W)
-        by_set = {}
-        for i, j in enumerate(jobs):
-            by_set.setdefault(j.set_id, []).append(i)
-        for sid, idx in by_set.items():
+        set_ids = np.array([j.set_id for j in jobs])
+        for sid in dict.fromkeys(set_ids.tolist()):
             ps = self.sets[sid]
-            idx = np.array(idx, dtype=np.int64)
+            idx = np.nonzero(set_ids == sid)[0]
+            Wj = words[idx]
             for kid, meta in enumerate(ps.metas):
                 C = max(1, meta["n_carry"])
                 n_out = meta.get("n_out", 1)
                 first = meta.get("first_group", kid)
-                T = np.array([n_tiles(meta, int(words[i])) for i in idx], dtype=np.int64)
+                s0 = 64 - meta["HR"]
+                T = np.where(Wj <= s0, 1, 1 + (Wj - s0 + meta["S"] - 1) // meta["S"]).astype(np.int64)
                 ent = np.zeros(len(idx), dtype=_JOB_DTYPE)
-                ent["n_words"] = words[idx]
+                ent["n_words"] = Wj
                 ent["group"] = first
                 ent["n_carry"] = C
                 ent["tile_start"] = np.concatenate([[0], np.cumsum(T)])[:-1]
                 ent["n_tiles"] = T
-                res_off = res_total + np.concatenate([[0], np.cumsum(words[idx] * n_out)])[:-1]
+                res_off = res_total + np.concatenate([[0], np.cumsum(Wj * n_out)])[:-1]
                 stat_off = stat_total + np.concatenate([[0], np.cumsum(T * C)])[:-1]
-                res_total += int(words[idx].sum()) * n_out
+                res_total += int(Wj.sum()) * n_out
                 stat_total += int((T * C).sum())
-                queues.append((sid, kid, idx, ent, res_off, stat_off, int(T.sum()), T))
-                rows.extend((int(i), first + j, int(r) + j * int(words[i]), int(words[i]))
-                            for i, r in zip(idx, res_off) for j in range(n_out))
+                queues.append((sid, kid, idx, ent, res_off, stat_off, int(T.sum()), T, first, n_out))
         res_total, stat_total = max(res_total, 1), max(stat_total, 1)
         # device buffers of this slot (grown if needed), then the tables that
         # point into them, packed into one byte image with 256-byte sections
@@ -252,7 +250,7 @@ class MatchServer:
         td["n_words"] = words
         td["word_start"] = np.concatenate([[0], np.cumsum(words)])[:-1]
         sections = [td.view(np.uint8)]
-        for (sid, kid, idx, ent, res_off, stat_off, total, T) in queues:
+        for (sid, kid, idx, ent, res_off, stat_off, total, T, first, n_out) in queues:
             ent["basis"] = basis.data_ptr() + 8 * basis_off[idx]
             ent["result"] = result.data_ptr() + 8 * res_off
             ent["status"] = status.data_ptr() + 4 * stat_off
@@ -281,7 +279,7 @@ class MatchServer:
             status[:stat_total].zero_()
             s = rt.current_stream_ptr(slot.stream)
             rt.transpose_batch(dbytes.data_ptr(), basis.data_ptr(), dmeta.data_ptr() + offs[0], n, int(words.sum()), s)
-            for q, (sid, kid, idx, ent, res_off, stat_off, total, T) in enumerate(queues):
+            for q, (sid, kid, idx, ent, res_off, stat_off, total, T, first, n_out) in enumerate(queues):
                 ps = self.sets[sid]
                 args = rt.BatchArgs()
                 args.jobs = dmeta.data_ptr() + offs[1 + 2 * q]
@@ -296,7 +294,12 @@ class MatchServer:
             slot.err_host.copy_(slot.ctl[:, rt.CTL_ERROR], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(slot.stream)
-        slot.event, slot.jobs, slot.out_rows = ev, jobs, rows
+        # per job: (result offset, first group, n_out) of each of its kernels
+        parts = [[] for _ in jobs]
+        for (sid, kid, idx, ent, res_off, stat_off, total, T, first, n_out) in queues:
+            for i, r in zip(idx.tolist(), res_off.tolist()):
+                parts[i].append((r, n_out))
+        slot.event, slot.jobs, slot.out_rows = ev, jobs, (parts, words)
         slot.t_launch = time.perf_counter()
         self.batches += 1
 
@@ -308,14 +311,13 @@ class MatchServer:
             if int(slot.err_host.abs().sum()) != 0:
                 raise rt.HipRuntimeError("carry look-back timed out in a served batch")
             out = slot.bufs["host_out"].numpy()
-            per_job = {}
-            for (i, g, off, W) in slot.out_rows:
-                per_job.setdefault(i, {})[g] = out[off:off + W]
+            parts, words = slot.out_rows
             now = time.perf_counter()
             self.batch_log.append((len(slot.jobs), slot.t_launch - slot.t_start, now - slot.t_launch))
             for i, job in enumerate(slot.jobs):
-                gs = per_job.get(i, {})
-                res = np.stack([gs[g] for g in sorted(gs)]).copy() if gs else np.zeros((0, 0), np.uint64)
+                W = int(words[i])
+                # kernels were queued in group order, so the blocks are too
+                res = np.concatenate([out[r:r + n * W].reshape(n, W) for r, n in parts[i]])
                 job.done = now
                 self.in_flight_bytes -= job.footprint
                 self.completed.append(job)
