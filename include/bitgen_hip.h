@@ -52,7 +52,9 @@ typedef struct bg_kernel_s *bg_kernel_t;
  * Field order is part of the ABI: generated kernels take it by value. */
 typedef struct bg_group_args {
   const uint64_t *basis; /* [8][n_words] basis streams, LSB-first words      */
-  uint64_t *result;      /* [n_words] match-end stream of the group          */
+  uint64_t *result;      /* [n_out][n_words] match-end streams of the
+                            kernel's groups (one group, or several merged
+                            consecutive groups: stream j at result + j*n_words) */
   uint32_t *status;      /* [n_tiles][n_carry] carry look-back status words,
                             zeroed before the first launch with this ctl     */
   uint32_t *ctl;         /* BG_CTL_WORDS words; ctl[BG_CTL_ERROR] is set
@@ -60,24 +62,26 @@ typedef struct bg_group_args {
                             (results are then invalid; the caller clears it) */
   uint64_t n_words;      /* words per stream                                 */
   uint32_t n_tiles;      /* tiles covering n_words                           */
-  uint32_t n_queues;     /* 0 = default; clamped to 1 .. min(32, blocks)     */
+  uint32_t n_queues;     /* 0 = default (64); clamped to 1 .. min(BG_MAX_QUEUES,
+                            blocks)                                          */
   uint32_t n_carry;      /* carry operations in the program (>= 1 slot)      */
   uint32_t take;         /* tiles per atomic take, 0 = default; 1 .. 8       */
 } bg_group_args_t;
 
-/* One (group, buffer) job of a batched launch.  Field order is ABI. */
+/* One (kernel, buffer) job of a batched launch.  Field order is ABI. */
 typedef struct bg_job {
   const uint64_t *basis; /* [8][n_words] basis of the buffer               */
-  uint64_t *result;      /* [n_words] match stream of (group, buffer)       */
+  uint64_t *result;      /* [n_out][n_words] match streams of the kernel's
+                            groups over the buffer                         */
   uint32_t *status;      /* [n_tiles][n_carry] look-back status words       */
   uint64_t n_words;
-  uint32_t group;        /* group program index inside the batch kernel     */
+  uint32_t group;        /* first group of the kernel (informational)       */
   uint32_t n_carry;      /* >= 1                                            */
   uint32_t tile_start;   /* first global tile index of this job             */
   uint32_t n_tiles;
 } bg_job_t;
 
-/* Launch descriptor of a generated batch kernel (all groups of a pattern set
+/* Launch descriptor of a generated batch kernel (one kernel of a pattern set
  * over many buffers).  Tiles are taken from a device queue: global tile index
  * g belongs to job tile_job[g]; jobs are ordered so each job's tiles are
  * contiguous and increasing. */
