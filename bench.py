@@ -337,7 +337,9 @@ def profiled_traffic(args, options):
             traffic = int(float(line.split("=")[1]))
         elif line.startswith("shape ="):
             shape = dict(kv.split("=") for kv in line.split("=", 1)[1].split())
-    want = {"groups_per_kernel": str(options.get("groups_per_kernel", 1)), "min_waves": str(options.get("min_waves", 0))}
+    want = {"groups_per_kernel": str(options.get("groups_per_kernel", 1)), "min_waves": str(options.get("min_waves", 0)),
+            "bigram_filter": str(options.get("bigram_filter", 0))}
+    shape = shape and {"bigram_filter": "0", **shape}
     return traffic if shape == want else None
 
 
@@ -376,7 +378,7 @@ def main():
         if not args.queues:
             args.queues = tuning["queues"]
         _log(f"autotuner: {tuning['options']} queues {tuning['queues']} "
-             f"({[(c['groups_per_kernel'], c['min_waves'], c['ms']) for c in tuning['candidates']]})")
+             f"({[(c['groups_per_kernel'], c['min_waves'], c.get('bigram_filter', 0), c['ms']) for c in tuning['candidates']]})")
     args.tuned_options = options
     cs = compiler.compile_pattern_set(groups, options)
     print(f"[bench] compiled {len(groups)} groups into {len(cs.metas)} kernels ({time.perf_counter() - t0:.1f}s)",

@@ -11,7 +11,10 @@ onto gfx950:
 * ``min_waves`` -- waves per SIMD the register allocator must leave room for
   (more waves hide more latency; past the program's natural register count
   values spill to scratch);
-* ``queues`` -- tile-queue counters the launch spreads its tile takes over.
+* ``queues`` -- tile-queue counters the launch spreads its tile takes over;
+* ``bigram_filter`` -- per-tile byte-pair filters in front of the literal
+  first stages of the regexes (a tile whose bytes hold none of a stage's
+  pairs skips that regex without evaluating its character classes).
 
 The result is cached under ``cache_dir/tune/<set digest>.json`` together
 with every candidate's time, keyed by the pattern set, the sample size, the
@@ -27,13 +30,15 @@ import time
 from . import config as cfg
 from . import compiler
 
-# candidate launch shapes: (groups_per_kernel, min_waves, queues)
+# candidate launch shapes: (groups_per_kernel, min_waves, queues, bigram_filter)
 CANDIDATES = [
-    (1, 0, 64),
-    (2, 5, 64),
-    (4, 5, 64),
-    (4, 0, 64),
-    (8, 5, 64),
+    (1, 0, 64, 0),
+    (2, 5, 64, 0),
+    (4, 5, 64, 0),
+    (4, 0, 64, 0),
+    (8, 5, 64, 0),
+    (1, 0, 64, 1),
+    (4, 5, 64, 1),
 ]
 
 
@@ -41,18 +46,18 @@ def _candidates(n_groups):
     """Distinct candidates for a set of n_groups groups (a merge wider than
     the set is the same kernel as merging all of it)."""
     seen, out = set(), []
-    for gpk, mw, q in CANDIDATES:
-        gpk = max(1, min(gpk, n_groups))
-        if (gpk, mw, q) not in seen:
-            seen.add((gpk, mw, q))
-            out.append((gpk, mw, q))
+    for gpk, mw, q, bg in CANDIDATES:
+        cand = (max(1, min(gpk, n_groups)), mw, q, bg)
+        if cand not in seen:
+            seen.add(cand)
+            out.append(cand)
     return out
 
 
 def options_of(candidate, base=None):
-    gpk, mw, _q = candidate
+    gpk, mw, _q, bg = candidate
     o = dict(base or {})
-    o.update(groups_per_kernel=gpk, min_waves=mw)
+    o.update(groups_per_kernel=gpk, min_waves=mw, bigram_filter=bg)
     return o
 
 
@@ -125,14 +130,16 @@ def tune(groups, sample, base_options=None, device=0, reps=5, use_cache=True):
         ks.check_errors()
         ms = e0.elapsed_time(e1) / reps
         info = ks.info()
-        rows.append({"groups_per_kernel": cand[0], "min_waves": cand[1], "queues": cand[2], "ms": round(ms, 4),
+        rows.append({"groups_per_kernel": cand[0], "min_waves": cand[1], "queues": cand[2],
+                     "bigram_filter": cand[3], "ms": round(ms, 4),
                      "kernels": ks.n_kernels, "vgprs": max(i["vgprs"] for i in info),
                      "scratch_bytes": max(i["scratch_bytes"] for i in info),
                      "setup_s": round(time.perf_counter() - t0, 2)})
         ks.free()
     best = min(rows, key=lambda r: r["ms"])
     out = {
-        "options": options_of((best["groups_per_kernel"], best["min_waves"], best["queues"]), base_options),
+        "options": options_of((best["groups_per_kernel"], best["min_waves"], best["queues"],
+                                best["bigram_filter"]), base_options),
         "queues": best["queues"],
         "ms": best["ms"],
         "sample_bytes": int(data.size),

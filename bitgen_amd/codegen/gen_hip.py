@@ -30,7 +30,7 @@ import math
 import os
 
 from .gen import Generator
-from ..inst import BsInstType
+from ..inst import BsInst, BsInstType
 from .. import config as cfg
 from ..csrc import build as native_build
 
@@ -80,11 +80,54 @@ def _cname(v):
     return f"v{v}"
 
 
+class BigramTest(BsInst):
+    """Zero test answered by the tile's byte-pair table (bg_bigram_build in
+    the device header): jump to ``label`` unless every clause of filter
+    ``slot`` may hold in the tile (a clause holds when one of its one or two
+    byte-pair hashes occurs at consecutive positions).  It is placed in front
+    of a first-stage zero test that stays in place behind it, so a tile
+    whose pairs are present (or collide) still reaches the exact tests."""
+
+    def __init__(self, slot, label, clauses):
+        super().__init__(BsInstType.SKIPZERO, [], None)
+        self.slot = slot
+        self.label = label
+        self.clauses = clauses
+
+    def uses(self):
+        return []
+
+    def __repr__(self):
+        return f"BigramTest({self.slot}:{self.clauses}->L{self.label})"
+
+
+_ALL_BYTES = (1 << 256) - 1
+_MOVABLE = (BsInstType.AND, BsInstType.OR, BsInstType.XOR, BsInstType.NOT, BsInstType.ASSIGN, BsInstType.ADVANCE)
+
+
+def _basis_set(i):
+    """Byte values whose basis-i bit (bit 7 - i of the byte) is set."""
+    m = 0
+    for b in range(256):
+        if (b >> (7 - i)) & 1:
+            m |= 1 << b
+    return m
+
+
+def _members(m):
+    return [b for b in range(256) if (m >> b) & 1]
+
+
+def bigram_hash(a, b):
+    """Same function as bg_bigram_hash in the device header."""
+    return ((a & 0x7F) << 8) | (b & 0xFF)
+
+
 class GroupProgram:
     """Analysis + HIP source of one group program."""
 
     def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
-                 min_waves=0):
+                 min_waves=0, bigram_filter=False):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -94,7 +137,10 @@ class GroupProgram:
         current regex's first zero test, so its instructions fill the
         latencies of the test and of the lane moves.  min_waves: waves per
         SIMD the register allocator must leave room for (0: its own choice;
-        a higher count may spill)."""
+        a higher count may spill).  bigram_filter: answer first-stage zero
+        tests of the form "S1 at p and S2 at p+1" (at most two byte pairs)
+        from a per-tile table of the byte pairs present, computing the stage
+        only when one of its pairs may occur."""
         if lookback not in ("speculate", "wait"):
             raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
         self.gid = gid
@@ -107,7 +153,24 @@ class GroupProgram:
         self.min_waves = int(min_waves)
         self.kernel_name = f"bg_group_{gid}"
         self._check()
+        self.bigram_keys = []
+        self.bigram_clauses = 0
+        if bigram_filter:
+            self._bigram_rewrite()
+        self._check_regions()
         self._halo()
+
+    def _check_regions(self):
+        """Carry ops never sit inside a skip range: every tile runs them, so
+        every tile publishes the carry status its successor waits for."""
+        open_ = set()
+        for inst in self.insts:
+            if inst.type == BsInstType.SKIPZERO:
+                open_.add(inst.label)
+            elif inst.type == BsInstType.LABEL:
+                open_.discard(inst.label)
+            elif inst.type in (BsInstType.MATCHSTAR, BsInstType.SCANTHRU) and open_:
+                raise HipCodegenError(f"group {self.gid}: carry operation {inst} inside a skip range")
 
     def _check(self):
         stores = 0
@@ -126,6 +189,201 @@ class GroupProgram:
         # one store per group: a program merged from several groups writes
         # their streams in order (result + j * W for the j-th store)
         self.n_out = stores
+
+    def _bigram_rewrite(self, max_pairs=2, max_clauses=3):
+        """Put a BigramTest in front of every top-level zero test whose skip
+        range has pair clauses, and move the instructions that only feed that
+        test's operand (and the skipped range) behind the BigramTest.
+
+        Position patterns: a single-definition value that is an AND of
+        position-local values (bitwise functions of the basis, each the set
+        of byte values for which it is 1) and of their advances or
+        look-aheads is nonzero at p only if the byte at p + o lies in S_o for
+        every offset o of its pattern {o: S_o}.  Values shifted in at the
+        window's edges are zero, so a nonzero value needs the bytes at p + o
+        and p + o + 1 (for consecutive offsets) inside the tile: the pair
+        clause (S_o x S_o+1) then holds in the tile's table.  The range of a
+        top-level zero test may be skipped when any of its tests would fail:
+        the test itself, and every nested test whose label closes together
+        with the range's and in front of which nothing defines a value used
+        after the range.  Each clause comes from one of those tests; up to
+        ``max_clauses`` clauses of at most ``max_pairs`` pairs each, most
+        selective first."""
+        insts = self.insts
+        ndefs, def_at, uses_at, labels, defs_of = {}, {}, {}, {}, {}
+        for k, inst in enumerate(insts):
+            for d in inst.defs():
+                ndefs[d] = ndefs.get(d, 0) + 1
+                def_at[d] = k
+                defs_of.setdefault(d, []).append(k)
+            for u in inst.uses():
+                uses_at.setdefault(u, []).append(k)
+            if inst.type == BsInstType.LABEL:
+                labels[inst.label] = k
+        sets, pats = {}, {}
+
+        def local(v):
+            c = self._const(v)
+            if c is not None:
+                return _ALL_BYTES if c == "1" else 0
+            return sets.get(v)
+
+        def pattern(v):
+            c = self._const(v)
+            if c is not None:
+                return {} if c == "1" else {0: 0}
+            return pats.get(v)
+
+        def inst_pattern(inst):
+            """Pattern of the value an AND / advance / assign computes."""
+            t = inst.type
+            if t == BsInstType.AND:
+                merged = {}
+                for u in inst.uses():
+                    for o, m in (pattern(u) or {}).items():
+                        merged[o] = merged.get(o, _ALL_BYTES) & m
+                return merged or None
+            if t == BsInstType.ADVANCE:
+                p = pattern(inst.operands[0])
+                if p:
+                    d = inst.amount if inst.is_left else -inst.amount
+                    return {o + d: m for o, m in p.items()}
+            elif t == BsInstType.ASSIGN:
+                p = pattern(inst.operands[0])
+                return dict(p) if p else None
+            return None
+
+        for inst in insts:
+            v = inst.ret
+            if v is None or ndefs.get(v) != 1:
+                continue
+            t = inst.type
+            ops = [local(u) for u in inst.uses()]
+            if t == BsInstType.STREAMLOAD:
+                sets[v] = _basis_set(int(inst.stream_index))
+            elif t in (BsInstType.AND, BsInstType.OR, BsInstType.XOR) and None not in ops:
+                a, b = ops
+                sets[v] = {BsInstType.AND: a & b, BsInstType.OR: a | b, BsInstType.XOR: a ^ b}[t]
+            elif t == BsInstType.NOT and None not in ops:
+                sets[v] = _ALL_BYTES ^ ops[0]
+            elif t == BsInstType.ASSIGN and None not in ops:
+                sets[v] = ops[0]
+            if v in sets:
+                pats[v] = {0: sets[v]}
+            else:
+                p = inst_pattern(inst)
+                if p:
+                    pats[v] = p
+
+        def clauses_of(v, p=None):
+            out = []
+            p = (pattern(v) if p is None else p) or {}
+            for o in sorted(p):
+                if o + 1 not in p:
+                    continue
+                a_set, b_set = _members(p[o]), _members(p[o + 1])
+                if not a_set or not b_set or len(a_set) * len(b_set) > max_pairs:
+                    continue
+                out.append(tuple(sorted({bigram_hash(a, b) for a in a_set for b in b_set})))
+            return out
+
+        plan = []   # (skip index, clauses, moved instruction indices)
+        open_ = set()
+        seg_start = 0
+        for k, inst in enumerate(insts):
+            if inst.type == BsInstType.LABEL:
+                open_.discard(inst.label)
+                if not open_:
+                    seg_start = k + 1
+                continue
+            if inst.type != BsInstType.SKIPZERO:
+                continue
+            top = not open_
+            open_.add(inst.label)
+            v = inst.operands[0]
+            if not top or inst.label not in labels:
+                continue
+            end = labels[inst.label]
+            found = list(clauses_of(v))
+            for n in range(k + 1, end):
+                nt = insts[n]
+                if nt.type != BsInstType.SKIPZERO or nt.label not in labels:
+                    continue
+                if any(insts[j].type != BsInstType.LABEL for j in range(labels[nt.label], end)):
+                    continue
+                # nothing in front of the nested test defines a value read
+                # after the range
+                if any(u >= end for j in range(k + 1, n) for d in insts[j].defs() for u in uses_at.get(d, ())):
+                    continue
+                found.extend(clauses_of(nt.operands[0]))
+            # the range's one result: zero before the range, computed once
+            # inside it; a clause of that computation that fails makes the
+            # result zero again, the value skipping leaves
+            outs = {d for j in range(k + 1, end) for d in insts[j].defs()
+                    if any(u > end for u in uses_at.get(d, ()))}
+            if len(outs) == 1:
+                x = outs.pop()
+                xdefs = [j for j, i2 in enumerate(insts) if x in i2.defs()]
+                if (len(xdefs) == 2 and seg_start <= xdefs[0] < k < xdefs[1] < end
+                        and insts[xdefs[0]].type == BsInstType.ASSIGN
+                        and self._const(insts[xdefs[0]].operands[0]) == "0"):
+                    found.extend(clauses_of(None, inst_pattern(insts[xdefs[1]])))
+            chosen = sorted(set(found), key=lambda c: (len(c), c))[:max_clauses]
+            if not chosen:
+                continue
+            # bitwise instructions and shifts that only feed v: move them
+            # behind the filter (carry ops stay: every tile publishes their
+            # carry status)
+            feed, stack = set(), [v]
+            while stack:
+                u = stack.pop()
+                j = def_at.get(u) if ndefs.get(u) == 1 else None
+                if j is None or j < seg_start or j >= k or j in feed or insts[j].type not in _MOVABLE:
+                    continue
+                # its inputs must hold the same values at the new place
+                if any(k2 > j for w in insts[j].uses() if ndefs.get(w, 0) > 1 for k2 in defs_of[w] if k2 < k):
+                    continue
+                feed.add(j)
+                stack.extend(insts[j].uses())
+            changed = True
+            while changed:
+                changed = False
+                for j in sorted(feed):
+                    ok = all(c in feed or c == k or k < c < end for c in uses_at.get(insts[j].ret, []))
+                    if not ok:
+                        feed.discard(j)
+                        changed = True
+            plan.append((k, chosen, sorted(feed)))
+        if not plan:
+            return
+        self.bigram_clauses = max(len(c) for _, c, _ in plan)
+        moved = {j for _, _, f in plan for j in f}
+        at = {k: (c, f) for k, c, f in plan}
+        out = []
+        for k, inst in enumerate(insts):
+            if k in moved:
+                continue
+            if k in at:
+                clauses, f = at[k]
+                out.append(BigramTest(len(self.bigram_keys), inst.label, clauses))
+                self.bigram_keys.append(clauses)
+                out.extend(insts[j] for j in f)
+            out.append(inst)
+        self.insts = out
+
+    def bigram_key_words(self):
+        """Filter f's clause c as one word (two 15-bit pair hashes, low and
+        high half, 0xFFFF = none; 0xFFFFFFFF = an unused clause that always
+        holds), bigram_clauses words per filter."""
+        words = []
+        for clauses in self.bigram_keys:
+            for c in range(self.bigram_clauses):
+                if c < len(clauses):
+                    h = clauses[c]
+                    words.append(h[0] | ((h[1] if len(h) > 1 else 0xFFFF) << 16))
+                else:
+                    words.append(0xFFFFFFFF)
+        return words
 
     def _const(self, v):
         n = self.vnm.get(v)
@@ -198,9 +456,14 @@ class GroupProgram:
         add = lines.append
         regex_note = json.dumps(self.regexes)[:2000].replace("*/", "* /")
         add(f"/* group {self.gid}: {regex_note} */")
+        nf = len(self.bigram_keys)
+        if nf:
+            words = self.bigram_key_words()
+            keys = ", ".join(f"0x{k:08X}u" for k in words)
+            add(f"static __constant__ uint32_t bg_keys_{self.gid}[{len(words)}] = {{{keys}}};")
         add(f"static __device__ __forceinline__ void bg_prog_{self.gid}(const uint64_t *__restrict__ basis, "
             "uint64_t *__restrict__ result, uint32_t *status, int64_t W, uint32_t t, uint32_t epoch, "
-            "uint32_t n_carry, uint32_t *err) {")
+            "uint32_t n_carry, uint32_t *err, uint32_t *bgt) {")
         add(f"  constexpr int HL = {self.HL}, HR = {self.HR}, S = {self.S};")
         add("  const int lane = bg_lane();")
         add("  const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;")
@@ -211,7 +474,7 @@ class GroupProgram:
             # every op waits for its predecessor (speculation state is kept in
             # 64-bit masks, so programs with more carry ops always wait)
             add("  tc.mode = BG_REPLAY;")
-        add("  (void)tc;")
+        add("  (void)tc; (void)bgt;")
         add("  uint64_t bas[8];")
         # all eight loads in flight at once: clamp the index instead of
         # predicating each load, then clear the lanes outside the stream
@@ -220,6 +483,11 @@ class GroupProgram:
         add("  for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + wc];")
         add("#pragma unroll")
         add("  for (int i = 0; i < 8; i++) bas[i] = inb ? bas[i] : 0ull;")
+        if nf:
+            # byte-pair table of the tile and the answers of every filter test
+            add("  bg_bigram_build(bas, bgt);")
+            add(f"  uint32_t bgm[{2 * ((nf + 63) // 64)}];")
+            add(f"  bg_bigram_lookup<{nf}, {self.bigram_clauses}>(bgt, bg_keys_{self.gid}, bgm);")
         # declarations (goto must not cross initialisations)
         vars_ = set()
         for inst in self.insts:
@@ -293,6 +561,8 @@ class GroupProgram:
             add = lines.append
             if speculative and idx == body_start:
                 add("  for (;;) {")
+                if self.bigram_keys:
+                    add("  " + " ".join(f"bg_sfence(bgm[{c}]);" for c in range(2 * ((len(self.bigram_keys) + 63) // 64))))
                 zeros = [f"{_cname(v)} = 0;" for v in body_vars]
                 for i in range(0, len(zeros), 16):
                     add("  " + " ".join(zeros[i:i + 16]))
@@ -321,6 +591,12 @@ class GroupProgram:
                 else:
                     add(f"  {{ int bg_unused = -1; {_cname(inst.ret)} = {fn}<{L}, S, {k}>("
                         f"{self.val(inst.operands[0])}, {self.val(inst.operands[1])}, tc, -1, bg_unused); }}")
+            elif isinstance(inst, BigramTest):
+                test = f"!(bgm[{inst.slot // 32}] & 0x{1 << (inst.slot % 32):X}u)"
+                if self.expect_zero:
+                    test = f"__builtin_expect({test}, 1)"
+                add(f"  if ({test}) goto L{inst.label};")
+                open_regions.add(inst.label)
             elif t == BsInstType.SKIPZERO:
                 test = f"bg_wave_zero({self.val(inst.operands[0])})"
                 if self.expect_zero:
@@ -597,6 +873,13 @@ class GroupProgram:
             return f"__launch_bounds__({block_threads}, {self.min_waves})"
         return f"__launch_bounds__({block_threads})"
 
+    def table_decl(self, block_threads):
+        """The wave's byte-pair table in LDS (4 KiB per wave), or none."""
+        if not self.bigram_keys:
+            return ["  uint32_t *bgt = nullptr;"]
+        return [f"  __shared__ uint32_t bg_tables[{block_threads // WAVE}][BG_BIGRAM_WORDS];",
+                "  uint32_t *bgt = bg_tables[threadIdx.x / BG_WAVE];"]
+
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
         return "\n".join([
@@ -606,13 +889,14 @@ class GroupProgram:
             "  // persistent: each wave takes tiles from its block's queue until the",
             "  // queue is exhausted",
             "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles, a.take);",
+            *self.table_decl(256),
             "  uint32_t first;",
             "  bool more = bg_queue_take(Q, first);",
             "  while (more) {",
             "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
             "    for (uint32_t i = 0, t = first; i < Q.take && t < Q.n; i++, t += Q.nq)",
             f"      bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
-            "a.ctl + BG_CTL_ERROR);",
+            "a.ctl + BG_CTL_ERROR, bgt);",
             "    more = bg_queue_finish(Q, next, first);",
             "  }",
             "  bg_queue_close(Q);",
@@ -629,6 +913,7 @@ class GroupProgram:
             "S": self.S,
             "n_carry": self.n_carry,
             "n_out": self.n_out,
+            "n_bigram_tests": len(self.bigram_keys),
             "n_insts": len(self.insts),
             "regexes": list(self.regexes),
         }
@@ -649,6 +934,7 @@ def batch_source(program, block_threads=256):
         program.device_function(),
         f'extern "C" __global__ {program.launch_bounds(block_threads)} void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
         "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles, a.take);",
+        *program.table_decl(block_threads),
         "  uint32_t first;",
         "  bool more = bg_queue_take(Q, first);",
         "  while (more) {",
@@ -657,7 +943,7 @@ def batch_source(program, block_threads=256):
         "      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
         "      const bg_job_t J = a.jobs[j];",
         f"      bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, Q.epoch, J.n_carry, "
-        "a.ctl + BG_CTL_ERROR);",
+        "a.ctl + BG_CTL_ERROR, bgt);",
         "    }",
         "    more = bg_queue_finish(Q, next, first);",
         "  }",

@@ -44,6 +44,8 @@ DEFAULT_OPTIONS = {
     # groups merged into one kernel (one launch computes all their streams
     # and shares their basis loads, class header, tile queue and look-back)
     "groups_per_kernel": 1,
+    # answer literal first-stage zero tests from a per-tile byte-pair table
+    "bigram_filter": 0,
 }
 
 
@@ -198,7 +200,7 @@ def program_for(gid, insts, vnm, regexes, options):
     """GroupProgram of one group's post-pass IR under the compile options."""
     return GroupProgram(gid, insts, vnm, regexes, lookback=options["lookback"],
                         expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]),
-                        min_waves=int(options["min_waves"]))
+                        min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]))
 
 
 @dataclass
@@ -226,6 +228,7 @@ def code_digest():
     if _code_digest is None:
         h = hashlib.sha256()
         paths = [os.path.join(_PKG, "compiler.py"), os.path.join(_PKG, "csrc", "build.py"),
+                 os.path.join(_PKG, "csrc", "transpose_core.h"),
                  os.path.join(_PKG, "template", "hip", "bitgen_device.h"),
                  os.path.join(os.path.dirname(_PKG), "include", "bitgen_hip.h")]
         for sub in ("tool", "bitparser", "inst", "passes", "codegen"):

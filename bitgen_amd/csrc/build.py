@@ -48,7 +48,8 @@ def kernel_flags():
 
 def _dep_digest():
     h = hashlib.sha256()
-    for path in (os.path.join(TEMPLATE, "bitgen_device.h"), os.path.join(INCLUDE, "bitgen_hip.h")):
+    for path in (os.path.join(TEMPLATE, "bitgen_device.h"), os.path.join(INCLUDE, "bitgen_hip.h"),
+                 os.path.join(CSRC, "transpose_core.h")):
         with open(path, "rb") as f:
             h.update(f.read())
     return h.hexdigest()

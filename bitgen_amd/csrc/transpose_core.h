@@ -56,3 +56,31 @@ BG_HD void bg_transpose_block(const uint64_t x_in[8], uint64_t y[8]) {
   BG_SWAP(x[6], x[7], 8, 0x00FF00FF00FF00FFull);
   for (int b = 0; b < 8; b++) y[7 - b] = x[b];
 }
+
+// Inverse of bg_transpose_block: basis words y[0..7] of a 64-position block
+// back to its bytes, x[j] = bytes 8j..8j+7 little-endian.  The byte
+// transpose and the in-word bit transpose are each their own inverse, so the
+// stages of bg_transpose_block run again in reverse order.
+BG_HD void bg_untranspose_block(const uint64_t y[8], uint64_t x_out[8]) {
+  uint64_t x[8];
+  for (int b = 0; b < 8; b++) x[b] = y[7 - b];
+  BG_SWAP(x[0], x[1], 8, 0x00FF00FF00FF00FFull);
+  BG_SWAP(x[2], x[3], 8, 0x00FF00FF00FF00FFull);
+  BG_SWAP(x[4], x[5], 8, 0x00FF00FF00FF00FFull);
+  BG_SWAP(x[6], x[7], 8, 0x00FF00FF00FF00FFull);
+  BG_SWAP(x[0], x[2], 16, 0x0000FFFF0000FFFFull);
+  BG_SWAP(x[1], x[3], 16, 0x0000FFFF0000FFFFull);
+  BG_SWAP(x[4], x[6], 16, 0x0000FFFF0000FFFFull);
+  BG_SWAP(x[5], x[7], 16, 0x0000FFFF0000FFFFull);
+  BG_SWAP(x[0], x[4], 32, 0x00000000FFFFFFFFull);
+  BG_SWAP(x[1], x[5], 32, 0x00000000FFFFFFFFull);
+  BG_SWAP(x[2], x[6], 32, 0x00000000FFFFFFFFull);
+  BG_SWAP(x[3], x[7], 32, 0x00000000FFFFFFFFull);
+  for (int j = 0; j < 8; j++) x_out[j] = bg_bit_transpose8(x[j]);
+}
+
+// Bigram filter hash: 15 bits of the byte pair (a, b) at positions (p, p+1):
+// the low 7 bits of a and all of b.  Pairs that differ only in bit 7 of a
+// share a slot (a superset filter, never a missed pair).
+BG_HD uint32_t bg_bigram_hash(uint32_t a, uint32_t b) { return ((a & 0x7Fu) << 8) | (b & 0xFFu); }
+

@@ -22,6 +22,7 @@
 #include <stdint.h>
 
 #include "bitgen_hip.h"
+#include "transpose_core.h"
 
 #define BG_WAVE 64
 
@@ -120,9 +121,89 @@ __device__ __forceinline__ uint64_t bg_advl(uint64_t x) {
 // boundaries so every regex recomputes its classes next to their use.
 __device__ __forceinline__ void bg_fence(uint64_t &x) { asm volatile("" : "+v"(x)); }
 
+// Scalar counterpart, for wave-uniform values: issued at the top of every
+// pass so that tests derived from x are not all hoisted in front of the
+// pass loop (where each would hold its own scalar registers).
+__device__ __forceinline__ void bg_sfence(uint64_t &x) { asm volatile("" : "+s"(x)); }
+__device__ __forceinline__ void bg_sfence(uint32_t &x) { asm volatile("" : "+s"(x)); }
+
 // ---- zero tests ------------------------------------------------------------
 __device__ __forceinline__ bool bg_wave_zero(uint64_t x) {
   return __builtin_amdgcn_ballot_w64(x != 0ull) == 0ull;
+}
+
+// ---- bigram filter -------------------------------------------------------------
+// A regex whose first zero test is "class S1 at p and class S2 at p+1" (a
+// literal's first two bytes) is nonzero in a tile only if one of its byte
+// pairs occurs in the tile.  Each wave builds, once per tile, a table of the
+// byte pairs of its window (15-bit hash, bg_bigram_hash: 32,768 bits = 4 KiB
+// of LDS per wave) and answers every such test with one table lookup, lane
+// f looking up test f; the answers become wave-uniform bit masks and a test
+// whose pairs are all absent jumps over the regex without computing it.  A
+// present pair (or a hash collision) falls through to the exact zero test,
+// so the table only has to hold every pair of the window (a superset is
+// fine), never exactly them.
+#define BG_BIGRAM_WORDS 1024
+
+// Table of the byte pairs (p, p+1) of the wave's window (bas = the tile's
+// basis words after the out-of-stream lanes are cleared).  Lane 63's last
+// position pairs with 0 (no lane above): one extra entry, harmless.
+__device__ __forceinline__ void bg_bigram_build(const uint64_t bas[8], uint32_t *tab) {
+  const int lane = bg_lane();
+  uint4 *t4 = reinterpret_cast<uint4 *>(tab);
+#pragma unroll
+  for (int i = 0; i < BG_BIGRAM_WORDS / 4 / BG_WAVE; i++) t4[lane + BG_WAVE * i] = make_uint4(0u, 0u, 0u, 0u);
+  uint64_t x[8];
+  bg_untranspose_block(bas, x);
+  // the byte after this lane's last position: the next lane's first byte
+  const uint32_t nxt = bg_next32<1>((uint32_t)x[0]);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const uint32_t lo = (uint32_t)x[j], hi = (uint32_t)(x[j] >> 32);
+    const uint32_t after = (j < 7) ? (uint32_t)x[j + 1] : nxt;
+    // v_perm_b32 bytes: selector 0-3 = lo, 4-7 = hi operand.  Each 32-bit
+    // result holds two 16-bit pair keys (second byte low, first byte high).
+    const uint32_t k01 = __builtin_amdgcn_perm(hi, lo, 0x01020001u) & 0x7FFF7FFFu;  // (b0,b1) (b1,b2)
+    const uint32_t k23 = __builtin_amdgcn_perm(hi, lo, 0x03040203u) & 0x7FFF7FFFu;  // (b2,b3) (b3,b4)
+    const uint32_t k45 = __builtin_amdgcn_perm(after, hi, 0x01020001u) & 0x7FFF7FFFu;  // (b4,b5) (b5,b6)
+    const uint32_t k67 = __builtin_amdgcn_perm(after, hi, 0x03040203u) & 0x7FFF7FFFu;  // (b6,b7) (b7,b8)
+    const uint32_t keys[8] = {k01 & 0xFFFFu, k01 >> 16, k23 & 0xFFFFu, k23 >> 16,
+                              k45 & 0xFFFFu, k45 >> 16, k67 & 0xFFFFu, k67 >> 16};
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      __hip_atomic_fetch_or(tab + (keys[k] >> 5), 1u << (keys[k] & 31u), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WAVEFRONT);
+  }
+}
+
+// Wave-uniform answers of NF filters of NCL clauses each: bit f % 32 of
+// masks[f / 32] is set when every clause of filter f may hold in the tile.
+// keys[f * NCL + c] holds clause c: up to two 15-bit pair hashes (low and
+// high half, 0xFFFF = none), one of which must be in the table;
+// 0xFFFFFFFF = an unused clause.  32-bit masks keep each test a single
+// scalar AND (its result sets SCC for the branch).
+template <int NF, int NCL>
+__device__ __forceinline__ void bg_bigram_lookup(const uint32_t *tab, const uint32_t *keys, uint32_t *masks) {
+  const int lane = bg_lane();
+#pragma unroll
+  for (int c = 0; c < (NF + 63) / 64; c++) {
+    const int f = c * 64 + lane;
+    bool hit = f < NF;
+    if (hit) {
+#pragma unroll
+      for (int j = 0; j < NCL; j++) {
+        const uint32_t k = keys[f * NCL + j];
+        if (k == 0xFFFFFFFFu) continue;
+        const uint32_t a = k & 0xFFFFu, b = k >> 16;
+        bool h = (tab[a >> 5] >> (a & 31u)) & 1u;
+        if (b != 0xFFFFu) h = h || ((tab[b >> 5] >> (b & 31u)) & 1u);
+        hit = hit && h;
+      }
+    }
+    const uint64_t m = __builtin_amdgcn_ballot_w64(hit);
+    masks[2 * c] = (uint32_t)m;
+    masks[2 * c + 1] = (uint32_t)(m >> 32);
+  }
 }
 
 // ---- three-input bitwise functions ---------------------------------------------
@@ -210,12 +291,20 @@ __device__ __forceinline__ int bg_first_from(uint64_t notp, int x) {
   return m ? (int)__builtin_ctzll(m) : 64;
 }
 
-// Look-back for the carry into op K of tile t: walk predecessors from t - 1
-// until an inclusive status is found (aggregates propagate; tile 0 is always
-// inclusive).  st0 is tile t - 1's status word if already loaded (else 0).
-// The wait is bounded so that a missing predecessor shows up as an error
-// flag instead of a hung wave.
+// Look-back for the carry into op K of tile t: the nearest predecessor whose
+// status is not an aggregate decides it (aggregates pass the carry on
+// unchanged; before tile 0 the carry is 0).  The predecessor is read (and
+// waited for) with one scalar load, as almost always it decides.  Behind an
+// aggregate the next 64 statuses are read at once (lane i reads tile j-1-i),
+// so a run of propagating tiles -- a long line under a line-end projection,
+// a star over a long run -- costs one load per 64 tiles; the aggregates
+// passed in the last step before an inclusive status get that inclusive
+// value written back (their own carry-out, the value they publish
+// themselves), which shortens the walks of later tiles.  st is tile t - 1's
+// status word if already loaded (else 0).  Waits are bounded so that a
+// missing predecessor shows up as an error flag instead of a hung wave.
 __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, uint32_t st) {
+  const uint32_t ep = tc.epoch & 0x0FFFFFFFu;
   int64_t j = (int64_t)tc.t - 1;
   uint32_t spins = 0;
   bool have = st != 0;
@@ -225,7 +314,7 @@ __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, u
     if (!have) st = (uint32_t)__builtin_amdgcn_readfirstlane((int)bg_st_load(tc.status + (size_t)j * tc.n_carry + K));
     have = false;
     const uint32_t kind = (st >> 2) & 3u;
-    if ((st >> 4) != (tc.epoch & 0x0FFFFFFFu) || kind == 0) {
+    if ((st >> 4) != ep || kind == 0) {
       if (++spins > BG_SPIN_LIMIT) {
         if (bg_lane() == 0) __hip_atomic_fetch_or(tc.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return 0;
@@ -234,9 +323,24 @@ __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, u
       continue;
     }
     if (kind == BG_ST_INC) return st & 1u;
-    // aggregate: the whole predecessor segment propagates; look further back
-    j -= 1;
-    if (j < 0) return 0;
+    // tile j passes its carry-in on: read the 64 tiles before it
+    const int lane = bg_lane();
+    const int64_t jj = j - 1 - lane;
+    uint32_t w = bg_st_pack(ep, BG_ST_INC, 0u);
+    if (jj >= 0) w = bg_st_load(tc.status + (size_t)jj * tc.n_carry + K);
+    const bool agg = (w >> 4) == ep && ((w >> 2) & 3u) == BG_ST_AGG;
+    const uint64_t stop = ~__builtin_amdgcn_ballot_w64(agg);
+    if (stop == 0) {
+      j -= 64;  // an aggregate too: read on from it
+      have = true;
+      continue;
+    }
+    const int f = (int)__builtin_ctzll(stop);
+    st = (uint32_t)__builtin_amdgcn_readlane((int)w, f);
+    j -= 1 + f;
+    have = true;
+    if ((st >> 4) == ep && ((st >> 2) & 3u) == BG_ST_INC && lane < f)
+      bg_st_store(tc.status + (size_t)jj * tc.n_carry + K, bg_st_pack(ep, BG_ST_INC, st & 1u));
   }
 }
 

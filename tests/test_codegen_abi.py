@@ -7,7 +7,7 @@ import subprocess
 
 import pytest
 
-from bitgen_amd import compiler
+from bitgen_amd import compiler, workloads
 from bitgen_amd.codegen import GroupProgram, HipCodegenError
 from bitgen_amd.csrc import build as nb
 
@@ -198,13 +198,49 @@ int main() {
     assert r.returncode == 0 and r.stdout.strip() == "0"
 
 
-def _interp(prog, data_bits, n, fused):
+def test_untranspose_core_host_build(tmp_path):
+    """The inverse block transpose (used by the bigram filter) gives back the
+    bytes for every block, and bg_bigram_hash keeps b and the low 7 bits of a."""
+    src = tmp_path / "u.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <cstdlib>
+#include "transpose_core.h"
+int main() {
+  srand(11); int bad = 0;
+  for (int it = 0; it < 5000; it++) {
+    uint64_t x[8], y[8], z[8];
+    for (int j = 0; j < 8; j++) x[j] = ((uint64_t)rand() << 40) ^ ((uint64_t)rand() << 20) ^ (uint64_t)rand();
+    bg_transpose_block(x, y);
+    bg_untranspose_block(y, z);
+    for (int j = 0; j < 8; j++) bad += (z[j] != x[j]);
+  }
+  for (unsigned a = 0; a < 256; a++)
+    for (unsigned b = 0; b < 256; b++)
+      bad += bg_bigram_hash(a, b) != (((a & 127u) << 8) | b);
+  printf("%d\n", bad); return bad != 0;
+}''')
+    exe = tmp_path / "u"
+    subprocess.run(["g++", "-O2", "-I", os.path.join(ROOT, "bitgen_amd", "csrc"), str(src), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "0"
+
+
+def _interp(prog, data_bits, n, fused, data_bytes=None):
     """Whole-stream interpreter of a generated program (Python ints as
     n-bit streams, bit p = position p).  With fused=True, bitwise
     instructions are evaluated through the generator's bitop3 trees and
-    absorbed instructions are skipped, as in the emitted code."""
+    absorbed instructions are skipped, as in the emitted code.  A bigram
+    test jumps when one of its clauses has none of its pair hashes among the
+    hashes of the byte pairs at consecutive positions of the whole stream
+    (positions past the data hold byte 0, as in the basis)."""
     from bitgen_amd.inst import BsInstType as T
-    from bitgen_amd.codegen.gen_hip import _LUT_IN, _lut, _leaves
+    from bitgen_amd.codegen.gen_hip import _LUT_IN, _lut, _leaves, BigramTest, bigram_hash
+
+    present = None
+    if data_bytes is not None:
+        padded = list(data_bytes) + [0] * max(0, n - len(data_bytes))
+        present = {bigram_hash(padded[q], padded[q + 1]) for q in range(n - 1)}
 
     mask = (1 << n) - 1
     trees, absorbed = prog._fuse()
@@ -260,6 +296,9 @@ def _interp(prog, data_bits, n, fused):
         elif t == T.SCANTHRU:
             m, c = val(inst.operands[0]), val(inst.operands[1])
             env[inst.ret] = ((m + c) & ~c) & mask
+        elif isinstance(inst, BigramTest):
+            if present is not None and not all(any(h in present for h in c) for c in inst.clauses):
+                skip_to = inst.label
         elif t == T.SKIPZERO:
             if val(inst.operands[0]) == 0:
                 skip_to = inst.label
@@ -309,3 +348,80 @@ def test_bitop3_lut_convention():
     assert _lut(("and", ("not", a), b), L) == 0x0C
     # a & ~b: rows with a=1, b=0 -> indices 4 and 5
     assert _lut(("and", a, ("not", b)), L) == 0x30
+
+
+LITERALS = ["hello", "abcz", "q0q0", "xyzzy", "a[bc]de", "[ab]cd", "[0-9]ab", "zz[^\\n]*yy", "ca.*bd", "ab",
+            "dcba(xy|zq)", "c?abc", "^abc", "xyz$", "bb[0-9]{2,4}c"]
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_bigram_filter_rewrite_is_exact(seed):
+    """The bigram rewrite (filter tests in front of literal first stages,
+    their stage instructions moved behind the filter) computes the oracle's
+    stream whether or not the filters jump, with and without the bitop3
+    trees; the filters' clauses come from the byte sets of the stages."""
+    import random
+    import numpy as np
+
+    from oracle import oracle as O
+    from bitgen_amd.tool.regex2pablo import regexes_to_pablo
+    from bitgen_amd.codegen.gen_hip import BigramTest
+
+    rng = random.Random(seed)
+    regs = rng.sample(LITERALS, rng.randint(2, 6))
+    n = rng.choice([40, 300, 2000])
+    alpha = b"abcdxyzq0123h\nelo"
+    data = bytearray(rng.choice(alpha) for _ in range(n))
+    for _ in range(rng.randint(0, 4)):
+        lit = rng.choice([b"hello", b"abcz", b"xyzzy", b"acde", b"q0q0", b"bb12c"])
+        off = rng.randrange(0, max(1, n - len(lit)))
+        data[off:off + len(lit)] = lit[:n - off]
+    data = np.frombuffer(bytes(data), dtype=np.uint8)
+    pre, pre_vnm, post, post_vnm = compiler.group_programs(regs)
+    prog = GroupProgram(0, post, post_vnm, regs, bigram_filter=True)
+    assert prog.bigram_keys and any(isinstance(i, BigramTest) for i in prog.insts)
+    basis = O.transpose(data)
+    npos = data.size + 1
+    ref = O.mask_to_positions(O.run(pre, pre_vnm, basis), npos)
+    W = basis.shape[1]
+    bits = [int.from_bytes(basis[i].tobytes(), "little") for i in range(8)]
+    for fused in (False, True):
+        got = _interp(prog, bits, W * 64, fused, data_bytes=data.tolist())
+        words = np.frombuffer(got.to_bytes(W * 8, "little"), dtype=np.uint64)
+        assert np.array_equal(O.mask_to_positions(words, npos), ref), (regs, fused)
+
+
+def test_bigram_filter_keeps_carry_ops_in_every_tile():
+    """A filter in front of a stage that follows a star must leave the star's
+    MatchStar ahead of it: a tile that skips a carry op never publishes the
+    carry status its successor waits for.  The generator checks that no
+    carry op sits inside a skip range; here on a star followed by a literal
+    and on the merged ids programs."""
+    from bitgen_amd.codegen.gen_hip import BigramTest
+    from bitgen_amd.inst import BsInstType as T
+
+    def carry_ops_in_ranges(insts):
+        open_, n = set(), 0
+        for i in insts:
+            if i.type == T.SKIPZERO:
+                open_.add(i.label)
+            elif i.type == T.LABEL:
+                open_.discard(i.label)
+            elif i.type in (T.MATCHSTAR, T.SCANTHRU) and open_:
+                n += 1
+        return n
+
+    regs = ["ca.*bd", "x[yz]q", "zz[^\\n]*yy"]
+    _, _, post, vnm = compiler.group_programs(regs)
+    prog = GroupProgram(0, post, vnm, regs, bigram_filter=True)
+    tests = [i for i in prog.insts if isinstance(i, BigramTest)]
+    assert len(tests) >= 2 and carry_ops_in_ranges(prog.insts) == 0
+    stars = [k for k, i in enumerate(prog.insts) if i.type == T.MATCHSTAR]
+    after_star = [k for k, i in enumerate(prog.insts) if isinstance(i, BigramTest) and k > stars[0]]
+    assert after_star, "a filter after the first star"
+    regexes, n_groups, _buf = workloads.ids(n_bytes=1 << 16, n_regex=1000, n_groups=16)
+    groups = compiler.group_regex_in_list(regexes, n_groups)
+    o = dict(compiler.DEFAULT_OPTIONS, groups_per_kernel=4, min_waves=5, bigram_filter=1)
+    for p in compiler.kernel_programs(groups[:8], o):
+        assert p.bigram_keys and carry_ops_in_ranges(p.insts) == 0
+

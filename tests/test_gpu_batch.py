@@ -86,3 +86,19 @@ def test_batched_merged_kernels(gpu):
     plan.destroy()
     for s in sets:
         s.free()
+
+
+def test_batched_bigram_filter(gpu):
+    """Batch kernels with byte-pair filters: the filter table is built per
+    tile from that tile's job, so jobs of other buffers never leak into it."""
+    sets = [BatchPatternSet(s, {"bigram_filter": 1, "groups_per_kernel": k}).load(0) for s, k in zip(SETS, (1, 2))]
+    buffers = _buffers(5, 12)
+    jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers))]
+    print(f"buffers={[b.size for b in buffers]}", flush=True)
+    plan = BatchPlan(sets, buffers, jobs)
+    plan.enqueue()
+    _check(plan, sets, buffers, jobs)
+    plan.destroy()
+    for s in sets:
+        s.free()
+

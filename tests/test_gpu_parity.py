@@ -58,6 +58,52 @@ def test_merged_group_kernels(gpu, seed, per_kernel):
                   label=f"merged seed {seed}")
 
 
+# regexes that open with a two-byte literal stage (bigram filters apply), and
+# some that do not
+BIGRAM_POOL = ["hello", "ab[0-9]+c", "x[yz]q", "q0q0", "[ab]cd", "zz[^\\n]*yy", "ca.*bd", "a.{0,3}b",
+               "dcba(xy|zq)", "c?abc", "^abc", "xyz$", "bb[0-9]{2,4}c", "\\x81\\x02", "[0-9]{2,4}"]
+
+
+@pytest.mark.parametrize("seed,per_kernel", [(s, k) for s in range(8) for k in (1, 2)])
+def test_bigram_filter_groups(gpu, seed, per_kernel):
+    """Per-tile byte-pair filters in front of literal first stages: tiles
+    that hold a stage's pair and tiles that do not (printable text with
+    planted literals, bytes above 127 whose pairs share a hash with ASCII
+    pairs), single and merged kernels, every group bit-exact."""
+    rng = random.Random(200 + seed)
+    g = np.random.default_rng(200 + seed)
+    n = rng.choice([60, 5000, 70000, 300001])
+    alpha = np.frombuffer(bytes(range(32, 127)) + b"\n\x81\x82\x02", dtype=np.uint8)
+    data = g.choice(alpha, size=n)
+    for _ in range(rng.randint(0, 12)):
+        lit = np.frombuffer(rng.choice([b"hello", b"ab12c", b"xyq", b"q0q0", b"acd", b"zzqyy", b"\x81\x02"]),
+                            dtype=np.uint8)
+        off = rng.randrange(0, max(1, n - lit.size))
+        data[off:off + lit.size] = lit[:n - off]
+    groups = [rng.sample(BIGRAM_POOL, rng.randint(1, 5)) for _ in range(rng.randint(1, 4))]
+    print(f"seed={seed} per_kernel={per_kernel} n={n} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, options={"bigram_filter": 1, "groups_per_kernel": per_kernel}, repeat=2,
+                  label=f"bigram seed {seed}")
+
+
+@pytest.mark.parametrize("per_kernel", [1, 2])
+def test_bigram_filter_after_star(gpu, per_kernel):
+    """Filters behind a star (the stage after `.*` tests a pair the text
+    rarely holds): the star's carry op still runs in every tile, so every
+    tile publishes its carry status and the match stretches across tiles."""
+    rng = np.random.default_rng(9)
+    n = 400_000
+    alpha = np.frombuffer(b"abcexyzq0123 ", dtype=np.uint8)  # no 'd', no LF
+    data = rng.choice(alpha, size=n)
+    for off, lit in ((1000, b"ca"), (150_000, b"bd"), (200_000, b"cab"), (390_000, b"xbd")):
+        data[off:off + len(lit)] = np.frombuffer(lit, dtype=np.uint8)
+    data[-1] = ord("\n")
+    groups = [["ca.*bd", "x[yz]q"], ["zz[^\\n]*yy", "q0q0"], ["a[bc]+e.*bd"]]
+    ref = assert_parity(groups, data, gpu, options={"bigram_filter": 1, "groups_per_kernel": per_kernel},
+                        repeat=2, label=f"bigram after star gpk {per_kernel}")
+    assert O.popcount(ref[0]) == 1
+
+
 def test_plain_small_workload(gpu):
     regexes, n_groups, buf = workloads.plain_small()
     from bitgen_amd.compiler import group_regex_in_list
@@ -78,6 +124,23 @@ def test_carry_crosses_many_tiles(gpu):
     groups = [["a.*b", "a[^\\n]*b"], ["ax*b"], ["q"]]
     ref = assert_parity(groups, body, gpu, label="long line")
     assert O.popcount(ref[0]) == 1 and O.popcount(ref[1]) == 1 and O.popcount(ref[2]) == 0
+
+
+@pytest.mark.parametrize("lookback", ["speculate", "wait"])
+def test_carry_runs_through_thousands_of_tiles(gpu, lookback):
+    """A 24 MiB line (no LF until the end): the line-end projection and the
+    stars propagate through ~6,000 tiles, so every tile's carry-in is decided
+    by a predecessor thousands of tiles back (the look-back reads 64 statuses
+    per step).  Matches start near the front, the middle and the end."""
+    n = 24 * 2**20
+    body = np.full(n, ord("x"), dtype=np.uint8)
+    body[[5, n // 2, n - 300]] = ord("a")
+    body[[n // 3, n - 10]] = ord("b")
+    body[-1] = ord("\n")
+    groups = [["a.*b", "a[^\n]*b"], ["ax*b"], ["q"], ["x{3}b"]]
+    print(f"n={n} lookback={lookback}", flush=True)
+    ref = assert_parity(groups, body, gpu, options={"lookback": lookback}, label=f"24 MiB line {lookback}")
+    assert [O.popcount(r) for r in ref] == [1, 1, 0, 1]
 
 
 def test_long_shifts_use_multi_word_halo(gpu):
