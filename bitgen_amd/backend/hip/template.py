@@ -76,7 +76,7 @@ class HipGroupSet:
         return 1 + (n_words - s0 + meta["S"] - 1) // meta["S"]
 
     def status_words(self, n_words):
-        return max(self.n_tiles(m, n_words) * max(1, m["n_carry"]) for m in self.metas)
+        return max(self.n_tiles(m, n_words) * m["status_slots"] for m in self.metas)
 
     def alloc_status(self, n_inputs, n_words):
         """Zeroed look-back status words [n_inputs, kernels, words] for buffers
@@ -99,7 +99,7 @@ class HipGroupSet:
         args.n_words = n_words
         args.n_tiles = self.n_tiles(meta, n_words)
         args.n_queues = self.n_queues
-        args.n_carry = max(1, meta["n_carry"])
+        args.n_carry = meta["status_slots"]
         args.take = self.take
         rt.group_launch(self.kernels[k], args, stream, self.grid_blocks)
 

@@ -143,7 +143,7 @@ class BatchPlan:
         for si, ps in enumerate(sets):
             bufs = [b for (s, b) in self.jobs if s == si]
             for kid, meta in enumerate(ps.metas):
-                C = max(1, meta["n_carry"])
+                C = meta["status_slots"]
                 n_out = meta.get("n_out", 1)
                 first = meta.get("first_group", kid)
                 entries, tile = [], 0

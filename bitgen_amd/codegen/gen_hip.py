@@ -127,7 +127,7 @@ class GroupProgram:
     """Analysis + HIP source of one group program."""
 
     def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
-                 min_waves=0, bigram_filter=False):
+                 min_waves=0, bigram_filter=False, replay_scope="stage"):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -140,7 +140,14 @@ class GroupProgram:
         a higher count may spill).  bigram_filter: answer first-stage zero
         tests of the form "S1 at p and S2 at p+1" (at most two byte pairs)
         from a per-tile table of the byte pairs present, computing the stage
-        only when one of its pairs may occur."""
+        only when one of its pairs may occur.  replay_scope: what a replay
+        pass recomputes -- "stage": from the first zero test or carry op on
+        (the basis and the class header stay in registers across passes),
+        "tile": the whole tile from its basis loads (nothing stays live
+        across the pass loop's back edge but the tile's carry state)."""
+        if replay_scope not in ("stage", "tile"):
+            raise ValueError(f"replay_scope must be 'stage' or 'tile', not {replay_scope!r}")
+        self.replay_scope = replay_scope
         if lookback not in ("speculate", "wait"):
             raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
         self.gid = gid
@@ -475,12 +482,26 @@ class GroupProgram:
             # 64-bit masks, so programs with more carry ops always wait)
             add("  tc.mode = BG_REPLAY;")
         add("  (void)tc; (void)bgt;")
-        add("  uint64_t bas[8];")
+        outs = [f"out{j}" for j in range(self.n_out)]
+        tile_scope = 0 < self.n_carry <= 64 and self.lookback == "speculate" and self.replay_scope == "tile"
         # all eight loads in flight at once: clamp the index instead of
         # predicating each load, then clear the lanes outside the stream
         add("  const int64_t wc = w < 0 ? 0 : (w >= W ? W - 1 : w);")
+        if tile_scope:
+            # the pass loop covers the whole tile: every value of the tile is
+            # declared (and so zeroed) inside it, and the load index passes
+            # through a fence so that nothing computed from the basis is
+            # hoisted out of the loop (and kept live across it)
+            add("  uint64_t " + ", ".join(f"{o} = 0" for o in outs) + ";")
+            add("  for (;;) {")
+            add("  " + " ".join(f"{o} = 0;" for o in outs))
+            add("  uint64_t wl = (uint64_t)wc;")
+            add("  bg_fence(wl);")
+        else:
+            add("  const uint64_t wl = (uint64_t)wc;")
+        add("  uint64_t bas[8];")
         add("#pragma unroll")
-        add("  for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + wc];")
+        add("  for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + (int64_t)wl];")
         add("#pragma unroll")
         add("  for (int i = 0; i < 8; i++) bas[i] = inb ? bas[i] : 0ull;")
         if nf:
@@ -500,8 +521,8 @@ class GroupProgram:
         for i in range(0, len(vars_), 16):
             chunk = ", ".join(f"{_cname(v)} = 0" for v in vars_[i:i + 16])
             add(f"  uint64_t {chunk};")
-        outs = [f"out{j}" for j in range(self.n_out)]
-        add("  uint64_t " + ", ".join(f"{o} = 0" for o in outs) + ";")
+        if not tile_scope:
+            add("  uint64_t " + ", ".join(f"{o} = 0" for o in outs) + ";")
         store_index = {}
         for inst in self.insts:
             if inst.type == BsInstType.STREAMSTORE:
@@ -525,6 +546,8 @@ class GroupProgram:
                 break
             body_start = first
         body_vars = list(dict.fromkeys(d for i in self.insts[body_start:] for d in i.defs()))
+        if tile_scope:
+            body_start = 0
         self.loop_start = body_start if speculative else None
         self.carry_bounds = self._carry_bounds(carry_index)
         # operand bound of carry op k = max over the carry ops j it depends on
@@ -560,13 +583,16 @@ class GroupProgram:
             lines = chunks[idx]
             add = lines.append
             if speculative and idx == body_start:
-                add("  for (;;) {")
+                if not tile_scope:
+                    add("  for (;;) {")
                 if self.bigram_keys:
                     add("  " + " ".join(f"bg_sfence(bgm[{c}]);" for c in range(2 * ((len(self.bigram_keys) + 63) // 64))))
-                zeros = [f"{_cname(v)} = 0;" for v in body_vars]
-                for i in range(0, len(zeros), 16):
-                    add("  " + " ".join(zeros[i:i + 16]))
-                add("  " + " ".join(f"{o} = 0;" for o in outs) + "".join(f" bg_bnd_{k} = -1;" for k in sorted(consumers)))
+                if not tile_scope:
+                    zeros = [f"{_cname(v)} = 0;" for v in body_vars]
+                    for i in range(0, len(zeros), 16):
+                        add("  " + " ".join(zeros[i:i + 16]))
+                    add("  " + " ".join(f"{o} = 0;" for o in outs)
+                        + "".join(f" bg_bnd_{k} = -1;" for k in sorted(consumers)))
             t = inst.type
             if t == BsInstType.STREAMLOAD:
                 add(f"  {_cname(inst.ret)} = bas[{int(inst.stream_index)}];")
@@ -912,6 +938,8 @@ class GroupProgram:
             "HR": self.HR,
             "S": self.S,
             "n_carry": self.n_carry,
+            # look-back status words per tile: one per carry op (>= 1)
+            "status_slots": max(1, self.n_carry),
             "n_out": self.n_out,
             "n_bigram_tests": len(self.bigram_keys),
             "n_insts": len(self.insts),

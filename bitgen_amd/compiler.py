@@ -46,6 +46,7 @@ DEFAULT_OPTIONS = {
     "groups_per_kernel": 1,
     # answer literal first-stage zero tests from a per-tile byte-pair table
     "bigram_filter": 0,
+    "replay_scope": "stage",
 }
 
 
@@ -200,7 +201,8 @@ def program_for(gid, insts, vnm, regexes, options):
     """GroupProgram of one group's post-pass IR under the compile options."""
     return GroupProgram(gid, insts, vnm, regexes, lookback=options["lookback"],
                         expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]),
-                        min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]))
+                        min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]),
+                        replay_scope=options["replay_scope"])
 
 
 @dataclass

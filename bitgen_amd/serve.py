@@ -220,7 +220,7 @@ class MatchServer:
             idx = np.nonzero(set_ids == sid)[0]
             Wj = words[idx]
             for kid, meta in enumerate(ps.metas):
-                C = max(1, meta["n_carry"])
+                C = meta["status_slots"]
                 n_out = meta.get("n_out", 1)
                 first = meta.get("first_group", kid)
                 s0 = 64 - meta["HR"]

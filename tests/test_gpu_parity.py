@@ -143,6 +143,22 @@ def test_carry_runs_through_thousands_of_tiles(gpu, lookback):
     assert [O.popcount(r) for r in ref] == [1, 1, 0, 1]
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_tile_scope_replays(gpu, seed):
+    """Replays that recompute the whole tile (basis loads, class header,
+    byte-pair table) instead of keeping the header live across passes:
+    star-heavy groups on long and short lines, merged and with filters."""
+    rng = random.Random(300 + seed)
+    alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
+    p = [0.95 / 12] * 12 + [0.05] if seed % 2 else [1 / 13] * 13
+    n = rng.choice([9000, 70000, 300001])
+    data = np.random.default_rng(300 + seed).choice(alpha, size=n, p=p)
+    groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(1, 4))]
+    opts = {"replay_scope": "tile", "groups_per_kernel": 1 + seed % 3, "bigram_filter": seed % 2}
+    print(f"seed={seed} n={n} opts={opts} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, options=opts, repeat=2, label=f"tile scope seed {seed}")
+
+
 def test_long_shifts_use_multi_word_halo(gpu):
     rng = np.random.default_rng(3)
     data = rng.choice(np.frombuffer(b"abcx\n", dtype=np.uint8), size=200_000, p=[.2, .2, .2, .39, .01])
