@@ -153,6 +153,31 @@ class MatchServer:
         self.inbox.put(job)
         return job.future
 
+    def warm(self, data):
+        """Before start(): run, on every slot, one batch holding one job per
+        pattern set over ``data`` and wait for it.  The first launch of every
+        kernel on every slot's stream is then paid before serving, and the
+        warm batches are not recorded."""
+        if self._thread is not None:
+            raise RuntimeError("warm() runs before start()")
+        self.torch.cuda.set_device(self.dev)
+        arr = np.ascontiguousarray(np.asarray(data, dtype=np.uint8))
+        for slot in self.slots:
+            jobs = []
+            for sid in self.sets:
+                job = MatchJob(sid, arr, time.perf_counter())
+                job.footprint = job_footprint(arr.size, self.sets[sid].n_groups)
+                self.in_flight_bytes += job.footprint
+                jobs.append(job)
+            self._launch(slot, jobs)
+            while slot.jobs is not None:
+                if not self._complete():
+                    time.sleep(50e-6)
+        self.completed.clear()
+        self.batch_log.clear()
+        self.batches = 0
+        return self
+
     def start(self):
         self._thread = threading.Thread(target=self._loop, name="bitgen-serve", daemon=True)
         self._thread.start()

@@ -162,17 +162,21 @@ __device__ __forceinline__ void bg_bigram_build(const uint64_t bas[8], uint32_t 
     const uint32_t lo = (uint32_t)x[j], hi = (uint32_t)(x[j] >> 32);
     const uint32_t after = (j < 7) ? (uint32_t)x[j + 1] : nxt;
     // v_perm_b32 bytes: selector 0-3 = lo, 4-7 = hi operand.  Each 32-bit
-    // result holds two 16-bit pair keys (second byte low, first byte high).
-    const uint32_t k01 = __builtin_amdgcn_perm(hi, lo, 0x01020001u) & 0x7FFF7FFFu;  // (b0,b1) (b1,b2)
-    const uint32_t k23 = __builtin_amdgcn_perm(hi, lo, 0x03040203u) & 0x7FFF7FFFu;  // (b2,b3) (b3,b4)
-    const uint32_t k45 = __builtin_amdgcn_perm(after, hi, 0x01020001u) & 0x7FFF7FFFu;  // (b4,b5) (b5,b6)
-    const uint32_t k67 = __builtin_amdgcn_perm(after, hi, 0x03040203u) & 0x7FFF7FFFu;  // (b6,b7) (b7,b8)
-    const uint32_t keys[8] = {k01 & 0xFFFFu, k01 >> 16, k23 & 0xFFFFu, k23 >> 16,
-                              k45 & 0xFFFFu, k45 >> 16, k67 & 0xFFFFu, k67 >> 16};
+    // result holds two 16-bit pair keys (second byte low, first byte high):
+    // (b0,b1) (b1,b2); (b2,b3) (b3,b4); (b4,b5) (b5,b6); (b6,b7) (b7,b8).
+    const uint32_t kk[4] = {__builtin_amdgcn_perm(hi, lo, 0x01020001u), __builtin_amdgcn_perm(hi, lo, 0x03040203u),
+                            __builtin_amdgcn_perm(after, hi, 0x01020001u),
+                            __builtin_amdgcn_perm(after, hi, 0x03040203u)};
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-      __hip_atomic_fetch_or(tab + (keys[k] >> 5), 1u << (keys[k] & 31u), __ATOMIC_RELAXED,
+    for (int q = 0; q < 4; q++) {
+      // word = hash >> 5 = bits 5..14 of the 16-bit key (the first byte's top
+      // bit, bit 15, is not part of the hash), bit = hash & 31
+      const uint32_t k = kk[q];
+      __hip_atomic_fetch_or(tab + __builtin_amdgcn_ubfe(k, 5u, 10u), 1u << (k & 31u), __ATOMIC_RELAXED,
                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+      __hip_atomic_fetch_or(tab + __builtin_amdgcn_ubfe(k, 21u, 10u), 1u << ((k >> 16) & 31u), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
   }
 }
 

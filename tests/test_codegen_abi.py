@@ -425,3 +425,17 @@ def test_bigram_filter_keeps_carry_ops_in_every_tile():
     for p in compiler.kernel_programs(groups[:8], o):
         assert p.bigram_keys and carry_ops_in_ranges(p.insts) == 0
 
+
+
+def test_bigram_table_slot_matches_hash():
+    """The table insert takes word = bits 5..14 and bit = bits 0..4 of the
+    16-bit key (first byte << 8 | second byte); that is the 15-bit pair hash
+    the filters are keyed on, for every byte pair (the first byte's top bit
+    is not part of the hash)."""
+    from bitgen_amd.codegen.gen_hip import bigram_hash
+
+    for a in range(256):
+        for b in range(256):
+            key = (a << 8) | b
+            h = bigram_hash(a, b)
+            assert (key >> 5) & 0x3FF == h >> 5 and key & 31 == h & 31

@@ -82,3 +82,20 @@ def test_admission_cap_queues_instead_of_failing(sets):
     with pytest.raises(ValueError):
         small.submit(0, datas[0]).result(timeout=10)
     small.stop()
+
+
+def test_warm_then_serve(sets):
+    """warm() runs one batch of every set on every slot before serving; the
+    warm batches are not recorded and later jobs are exact."""
+    rng = random.Random(8)
+    g = np.random.default_rng(8)
+    srv = MatchServer(sets, n_streams=3, max_batch_jobs=8).warm(_data(rng, g))
+    assert srv.batches == 0 and not srv.completed and srv.in_flight_bytes == 0
+    srv.start()
+    jobs = [(rng.choice(list(SETS)), _data(rng, g)) for _ in range(30)]
+    futs = [srv.submit(sid, d) for sid, d in jobs]
+    results = [f.result(timeout=60) for f in futs]
+    srv.stop()
+    assert len(srv.completed) == len(jobs)
+    for (sid, d), r in zip(jobs, results):
+        _check(sid, d, r)
