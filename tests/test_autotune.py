@@ -1,0 +1,29 @@
+"""Autotuner bookkeeping on the CPU: candidate shapes, the options each
+stands for, and the cache key (timing itself needs a GPU)."""
+from bitgen_amd import autotune, compiler
+
+
+def test_candidates_are_distinct_and_clamped():
+    one = autotune._candidates(1)
+    assert len(one) == len(set(one))
+    assert all(c[0] == 1 for c in one)  # a merge wider than the set is the whole set
+    many = autotune._candidates(16)
+    assert (4, 5, 64, 1) in many and (1, 0, 64, 0) in many
+    assert len(many) == len(autotune.CANDIDATES)
+
+
+def test_options_of_candidate_keep_the_base():
+    o = autotune.options_of((8, 5, 64, 1), {"lookback": "speculate", "min_waves": 2})
+    assert o == {"lookback": "speculate", "groups_per_kernel": 8, "min_waves": 5, "bigram_filter": 1}
+    for key in ("groups_per_kernel", "min_waves", "bigram_filter"):
+        assert key in compiler.DEFAULT_OPTIONS
+
+
+def test_cache_key_depends_on_set_sample_device_and_code():
+    g = [["ab", "c.*d"], ["xyz"]]
+    k = autotune._key(g, None, 1 << 20, "dev")
+    assert k == autotune._key([list(x) for x in g], None, 1 << 20, "dev")
+    assert k != autotune._key(g, None, 2 << 20, "dev")
+    assert k != autotune._key(g, None, 1 << 20, "other")
+    assert k != autotune._key(g[:1], None, 1 << 20, "dev")
+    assert k != autotune._key(g, {"bigram_filter": 1}, 1 << 20, "dev")
