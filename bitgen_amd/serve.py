@@ -12,6 +12,7 @@ so it does not depend on which other jobs shared the batch.
 
 The reference has no serving component; this is the north star's addition.
 """
+import gc
 import queue
 import threading
 import time
@@ -110,14 +111,17 @@ class _Slot:
 
 class MatchServer:
     def __init__(self, pattern_sets, device=0, hbm_cap_bytes=200 * 10**9, max_batch_jobs=512,
-                 max_batch_bytes=256 * 2**20, n_streams=3, grid_waves_per_cu=16):
-        """pattern_sets: {set_id: loaded BatchPatternSet}."""
+                 max_batch_bytes=256 * 2**20, n_streams=3, grid_waves_per_cu=16, freeze_gc=True):
+        """pattern_sets: {set_id: loaded BatchPatternSet}.  freeze_gc: move the
+        objects alive at start() out of the garbage collector's reach while
+        serving (gc.freeze; undone by stop())."""
         import torch
 
         self.torch = torch
         self.dev = torch.device("cuda", device)
         rt.init(device)
         self.sets = dict(pattern_sets)
+        self.freeze_gc = bool(freeze_gc)
         self.cap = int(hbm_cap_bytes)
         self.max_batch_jobs = max_batch_jobs
         self.max_batch_bytes = max_batch_bytes
@@ -179,6 +183,13 @@ class MatchServer:
         return self
 
     def start(self):
+        # a full collection of the interpreter's heap (torch alone holds
+        # hundreds of thousands of objects) stalls the scheduler thread for
+        # tens of milliseconds; objects alive at start move to the permanent
+        # generation, so later collections only scan what serving creates
+        if self.freeze_gc:
+            gc.collect()
+            gc.freeze()
         self._thread = threading.Thread(target=self._loop, name="bitgen-serve", daemon=True)
         self._thread.start()
         return self
@@ -190,6 +201,8 @@ class MatchServer:
         self._stop.set()
         if self._thread:
             self._thread.join()
+        if self.freeze_gc:
+            gc.unfreeze()
 
     # -- scheduler -----------------------------------------------------------------
     def _loop(self):
