@@ -39,7 +39,6 @@ CANDIDATES = [
     (8, 5, 64, 0),
     (1, 0, 64, 1),
     (4, 5, 64, 1),
-    (8, 5, 64, 1),
 ]
 
 

@@ -142,6 +142,18 @@ class MatchServer:
         self._thread = None
         props = torch.cuda.get_device_properties(self.dev)
         self.grid_waves = props.multi_processor_count * grid_waves_per_cu
+        # Batches on different slots run at the same time.  A persistent
+        # launch needs a resident block for every one of its tile queues (a
+        # tile waits for its predecessor, which only a resident block of the
+        # predecessor's queue can take), and two launches that each fill part
+        # of the device can starve each other.  So each launch takes at most
+        # its 1/n_streams share of the blocks one CU can hold of that kernel:
+        # all the slots' launches then fit on the device together.
+        self.grid_blocks_max = {}
+        for sid, ps in self.sets.items():
+            for kid, kern in enumerate(ps.kernels):
+                per_cu, cus = rt.kernel_occupancy(kern, ps.block_threads)
+                self.grid_blocks_max[(sid, kid)] = max(1, per_cu // max(1, n_streams)) * max(1, cus)
 
     # -- client side -------------------------------------------------------------
     def submit(self, set_id, data):
@@ -326,7 +338,8 @@ class MatchServer:
                 args.n_jobs = len(idx)
                 args.total_tiles = total
                 wpb = ps.block_threads // 64
-                blocks = max(1, min(self.grid_waves // wpb, (total + wpb - 1) // wpb))
+                blocks = max(1, min(self.grid_waves // wpb, self.grid_blocks_max[(sid, kid)],
+                                    (total + wpb - 1) // wpb))
                 rt.batch_launch(ps.kernels[kid], args, blocks, ps.block_threads, s)
             host_out[:res_total].copy_(result[:res_total], non_blocking=True)
             slot.err_host.copy_(slot.ctl[:, rt.CTL_ERROR], non_blocking=True)
