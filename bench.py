@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--served-jobs", type=int, default=3000, help="jobs per arrival rate")
     ap.add_argument("--served-rates", default="2000,10000,50000", help="Poisson arrival rates (jobs/s), comma separated")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--genomic-mib", type=int, default=1024, help="DNA text of the genomic line (MiB)")
+    ap.add_argument("--no-genomic", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (three times) and exit: the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -121,6 +123,57 @@ def cpu_baseline(groups, buf, sample_bytes):
 
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def genomic_measurement(args, steps=3):
+    """Genomic motif scan: 512 IUPAC motifs (half gapped) in 32 groups over
+    synthetic DNA (1 GiB by default), plain form, default compile options
+    (one kernel per group); whole-pass GB/s over the resident text."""
+    import numpy as np
+    import torch
+
+    from bitgen_amd import compiler, workloads
+    from bitgen_amd.bitstream import transpose_bytes
+
+    t0 = time.perf_counter()
+    n_bytes = args.genomic_mib * 2**20
+    regexes, n_groups, buf = workloads.genomic(n_bytes=n_bytes)
+    groups = compiler.group_regex_in_list(regexes, n_groups)
+    ks = compiler.compile_pattern_set(groups).load(0)
+    dev = torch.device("cuda", 0)
+    staged = torch.from_numpy(buf).to(dev).reshape(1, 1, -1)
+    del buf
+    basis, staged = transpose_bytes(staged, n_bytes)
+    W = basis.size(-1)
+    result = torch.empty((1, 1, ks.n_groups, W), dtype=torch.uint64, device=dev)
+    status = ks.alloc_status(1, W)
+    _log(f"genomic workload ready: {len(regexes)} regexes, {ks.n_groups} groups, {n_bytes} bytes "
+         f"({time.perf_counter() - t0:.1f}s)")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        transpose_bytes(staged, n_bytes, out=basis)
+        ks.run(basis, result, status)
+
+    step()
+    torch.cuda.synchronize()
+    ks.check_errors()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+    for e in ev:
+        e[0].record(stream)
+        step()
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    ks.check_errors()
+    ms = float(np.median([e[0].elapsed_time(e[1]) for e in ev]))
+    out = {"workload": "genomic", "regexes": len(regexes), "groups": ks.n_groups, "kernels": ks.n_kernels,
+           "text_bytes": n_bytes, "ms_per_pass": round(ms, 3), "gbs": round(n_bytes / (ms / 1e3) / 1e9, 3),
+           "options": "default (one kernel per group)", "steps": steps}
+    _log(f"genomic: {out}")
+    ks.free()
+    del basis, staged, result, status
+    torch.cuda.empty_cache()
+    return out
 
 
 def batched_measurement(args):
@@ -489,6 +542,8 @@ def main():
     if count is not None:
         out["match_count"] = count
     ks.free()
+    if not args.no_genomic:
+        out["genomic"] = genomic_measurement(args)
     if not args.no_batched:
         out["batched"] = batched_measurement(args)
     if not args.no_served:
