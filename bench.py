@@ -127,8 +127,9 @@ def _log(msg):
 
 def genomic_measurement(args, steps=3):
     """Genomic motif scan: 512 IUPAC motifs (half gapped) in 32 groups over
-    synthetic DNA (1 GiB by default), plain form, default compile options
-    (one kernel per group); whole-pass GB/s over the resident text."""
+    synthetic DNA (1 GiB by default), plain form, launch shape from the
+    autotuner (on a sample of the text); whole-pass GB/s over the resident
+    text."""
     import numpy as np
     import torch
 
@@ -139,7 +140,15 @@ def genomic_measurement(args, steps=3):
     n_bytes = args.genomic_mib * 2**20
     regexes, n_groups, buf = workloads.genomic(n_bytes=n_bytes)
     groups = compiler.group_regex_in_list(regexes, n_groups)
-    ks = compiler.compile_pattern_set(groups).load(0)
+    options, tuning = {}, None
+    if not args.no_tune:
+        from bitgen_amd import autotune
+
+        tuning = autotune.tune(groups, buf[:args.tune_sample_mib * 2**20])
+        options = dict(tuning["options"])
+    ks = compiler.compile_pattern_set(groups, options).load(0)
+    if tuning:
+        ks.n_queues = tuning["queues"]
     dev = torch.device("cuda", 0)
     staged = torch.from_numpy(buf).to(dev).reshape(1, 1, -1)
     del buf
@@ -168,7 +177,9 @@ def genomic_measurement(args, steps=3):
     ms = float(np.median([e[0].elapsed_time(e[1]) for e in ev]))
     out = {"workload": "genomic", "regexes": len(regexes), "groups": ks.n_groups, "kernels": ks.n_kernels,
            "text_bytes": n_bytes, "ms_per_pass": round(ms, 3), "gbs": round(n_bytes / (ms / 1e3) / 1e9, 3),
-           "options": "default (one kernel per group)", "steps": steps}
+           "options": options or "default (one kernel per group)", "steps": steps,
+           "autotune": [(c["groups_per_kernel"], c["min_waves"], c.get("bigram_filter", 0), c["ms"])
+                        for c in tuning["candidates"]] if tuning else None}
     _log(f"genomic: {out}")
     ks.free()
     del basis, staged, result, status
