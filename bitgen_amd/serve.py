@@ -151,6 +151,8 @@ class MatchServer:
         # all the slots' launches then fit on the device together.
         self.grid_blocks_max = {}
         for sid, ps in self.sets.items():
+            if ps.kernels is None:
+                raise ValueError(f"pattern set {sid} is not loaded (call BatchPatternSet.load first)")
             for kid, kern in enumerate(ps.kernels):
                 per_cu, cus = rt.kernel_occupancy(kern, ps.block_threads)
                 self.grid_blocks_max[(sid, kid)] = max(1, per_cu // max(1, n_streams)) * max(1, cus)
