@@ -65,6 +65,9 @@ def admit(waiting, in_flight_bytes, cap, max_jobs, max_bytes):
     return batch, rest, in_flight_bytes
 
 
+_ZEROS = np.zeros(64, dtype=np.uint8)
+
+
 class _Slot:
     """One stream with its in-flight batch and its own buffers, reused from
     batch to batch and grown (never shrunk) when a batch needs more: pinned
@@ -315,8 +318,16 @@ class MatchServer:
         meta_bytes = max(o, 256)
         host_in = slot.buf("host_in", total_bytes, torch.uint8, pinned=True)
         hv = host_in.numpy()
-        for j, off in zip(jobs, byte_off):
-            hv[off:off + j.data.size] = j.data
+        # one concatenation into the pinned buffer: each job's bytes, then
+        # zeros up to its 64-byte boundary
+        pad = padded - lengths
+        pieces = []
+        for j, p in zip(jobs, pad.tolist()):
+            pieces.append(j.data)
+            if p:
+                pieces.append(_ZEROS[:p])
+        if pieces:
+            np.concatenate(pieces, out=hv[:int(padded.sum())])
         host_meta = slot.buf("host_meta", meta_bytes, torch.uint8, pinned=True)
         mv = host_meta.numpy()
         for sec, off in zip(sections, offs):
