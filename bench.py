@@ -405,8 +405,8 @@ def profiled_traffic(args, options):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "but not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; sharding over several GPUs is designed "
+                                   "(by group set or input range) but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
