@@ -33,6 +33,7 @@ Two more measurements ride on the same line (the metric names them):
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -57,17 +58,22 @@ def parse():
                     help="compiler option override (e.g. pass_short_circuit_interval=2); skips the autotuner")
     ap.add_argument("--no-tune", action="store_true", help="default launch shape instead of the autotuner's")
     ap.add_argument("--tune-sample-mib", type=int, default=16, help="sample the autotuner times candidates on")
-    ap.add_argument("--cpu-sample-mib", type=int, default=8)
+    ap.add_argument("--cpu-sample-mib", type=int, default=4,
+                    help="prefix of the IDS buffer the CPU baseline scans (per run; 2 warm-ups + 3 timed runs, "
+                         "at 1 thread and at all cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", action="store_true", help="check the GPU result against the oracle on a slice")
     ap.add_argument("--batched-buffers", type=int, default=4096)
-    ap.add_argument("--batched-mib", type=int, default=512, help="total bytes of the batched workload's buffers")
+    ap.add_argument("--batched-mib", type=int, default=2048, help="total bytes of the batched workload's buffers")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--served-jobs", type=int, default=3000, help="jobs per arrival rate")
-    ap.add_argument("--served-rates", default="2000,10000,50000", help="Poisson arrival rates (jobs/s), comma separated")
+    ap.add_argument("--served-jobs", type=int, default=10000, help="jobs per arrival rate")
+    ap.add_argument("--served-rates", default="2000,5000,10000,20000,50000",
+                    help="Poisson arrival rates (jobs/s), comma separated")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--genomic-mib", type=int, default=1024, help="DNA text of the genomic line (MiB)")
     ap.add_argument("--no-genomic", action="store_true")
+    ap.add_argument("--prepare", action="store_true",
+                    help="compile every code object a default run loads (no GPU needed) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (three times) and exit: the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -89,35 +95,95 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_bytes": n}), flush=True)
 
 
-def cpu_baseline(groups, buf, sample_bytes):
-    """Oracle (pre-pass IR, whole-stream C executor) over a prefix sample,
-    one thread per group (ctypes releases the GIL)."""
+def prepare(args):
+    """Compile (hipcc only, no GPU) every code object a default run loads:
+    each autotuner candidate of the IDS and genomic sets, and the batched and
+    served sets in batch and plain form under the autotuner's usual choice
+    (other choices compile when first loaded)."""
     import concurrent.futures
+
+    from bitgen_amd import autotune, compiler, workloads
+    from bitgen_amd.batch import BatchPatternSet
+
+    t0 = time.perf_counter()
+    ids_groups = compiler.group_regex_in_list(workloads.ids_regexes(args.regexes, 3), args.groups)
+    gen_groups = compiler.group_regex_in_list(workloads.genomic_regexes(512, 5), 32)
+    opts = autotune.options_of(autotune.USUAL)
+    tasks = [(compiler.compile_pattern_set, g, autotune.options_of(c))
+             for g in (ids_groups, gen_groups) for c in autotune.candidates(len(g))]
+    for regs in workloads.batched_sets() + workloads.served_sets():
+        g = compiler.group_regex_in_list(regs, workloads.groups_for(len(regs)))
+        tasks += [(compiler.compile_pattern_set, g, opts), (BatchPatternSet, g, opts)]
+    with concurrent.futures.ThreadPoolExecutor(max_workers=3) as ex:
+        for f in [ex.submit(fn, g, o) for fn, g, o in tasks]:
+            f.result()
+    _log(f"prepared {len(tasks)} pattern-set forms ({time.perf_counter() - t0:.0f}s)")
+
+
+def _cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    try:
+        return int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(groups, buf, sample_bytes):
+    """The oracle (C executor of the same pre-pass IR, whole streams) over a
+    prefix of the buffer, timed like the reference times its CPU path
+    (scripts/run_icgrep.py:20-21: 2 warm-ups, then the median of 3 runs), at
+    one thread and with one thread per group over all host threads (ctypes
+    releases the GIL, so the groups run in parallel)."""
+    import concurrent.futures
+
+    import numpy as np
 
     from bitgen_amd import compiler
     from oracle import oracle as O
 
-    sample = buf[:sample_bytes]
+    sample = np.ascontiguousarray(buf[:sample_bytes])
     W = O.n_words_for(sample.size)
     basis = O.transpose(sample, W)
     progs = [O.flatten(*compiler.group_programs(g)[:2]) for g in groups]
-    threads = min(len(progs), 16, os.cpu_count() or 1)
+    nproc = _host_threads()
 
-    def one(p):
-        return O.run_records(p[0], p[1], basis)
+    def timed(threads):
+        def one(p):
+            return O.run_records(p[0], p[1], basis)
 
-    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
-        list(ex.map(one, progs[:1]))  # warm-up
-        t0 = time.perf_counter()
-        list(ex.map(one, progs))
-        dt = time.perf_counter() - t0
+        runs = []
+        with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+            for r in range(5):
+                t0 = time.perf_counter()
+                list(ex.map(one, progs))
+                if r >= 2:
+                    runs.append(time.perf_counter() - t0)
+        return float(np.median(runs))
+
+    all_threads = min(nproc, len(progs))
+    t1 = timed(1)
+    tn = timed(all_threads)
     return {
-        "value": round(sample.size / dt / 1e9, 6),
+        "value": round(sample.size / tn / 1e9, 6),
         "unit": "GB/s",
-        "cores": threads,
+        "cores": all_threads,
         "kind": "port",
-        "sample": f"{sample.size / 2**20:.0f} MiB prefix of the same buffer, all {len(groups)} groups, "
-                  f"oracle C executor, one thread per group ({dt:.2f} s wall)",
+        "value_1_thread": round(sample.size / t1 / 1e9, 6),
+        "host_threads": nproc,
+        "cpu_model": _cpu_model(),
+        "sample": f"{sample.size / 2**20:.0f} MiB prefix of the IDS buffer, all {len(groups)} groups; oracle C "
+                  f"executor of the pre-pass IR; 2 warm-ups + median of 3 runs; {all_threads} threads "
+                  f"(one per group) {tn:.3f} s per run, 1 thread {t1:.3f} s per run",
     }
 
 
@@ -223,6 +289,41 @@ def batched_measurement(args):
     plan.check_errors()
     batched_ms = e0.elapsed_time(e1) / reps
     _log(f"batched: {batched_ms:.3f} ms per batch")
+    # per-launch durations (eager, HIP events on the launch stream between
+    # the launches the graph holds) for the roofline of the batch kernels:
+    # algorithmic bytes of a (set, kernel) launch = its jobs' basis reads
+    # (N bytes each) + their match streams (n_out * 8 W bytes each)
+    sptr_b = rt.current_stream_ptr(stream)
+    n_launch = len(plan.queues)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2 + n_launch)] for _ in range(3)]
+    for ev in evs:
+        ev[0].record(stream)
+        plan.enqueue_transpose(sptr_b)
+        ev[1].record(stream)
+        for i, q in enumerate(plan.queues):
+            ps = plan.sets[q["set"]]
+            if q["total_tiles"]:
+                rt.batch_launch(ps.kernels[q["kernel"]], q["args"], plan._grid(ps, q["total_tiles"]),
+                                ps.block_threads, sptr_b)
+            ev[2 + i].record(stream)
+    torch.cuda.synchronize()
+    plan.check_errors()
+    launch_ms = [float(np.median([ev[1 + i].elapsed_time(ev[2 + i]) for ev in evs])) for i in range(n_launch)]
+    alg = [sum(plan.lengths[e["buffer"]] + e["n_out"] * 8 * e["W"] for e in q["entries"]) for q in plan.queues]
+    dom = int(np.argmax(launch_ms))
+    batch_roofline = {
+        "bound": "hbm",
+        "achieved": round(sum(alg) / (sum(launch_ms) / 1e3) / 1e9, 3),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(sum(alg) / (sum(launch_ms) / 1e3) / 1e9 / HBM_PEAK_GBS, 6),
+        "traffic": None,
+        "kernels": "all batch launches of the replay (sum of bytes / sum of durations)",
+        "dominant": {"set": plan.queues[dom]["set"], "kernel": plan.queues[dom]["kernel"],
+                     "ms": round(launch_ms[dom], 4), "alg_bytes": int(alg[dom]),
+                     "frac": round(alg[dom] / (launch_ms[dom] / 1e3) / 1e9 / HBM_PEAK_GBS, 6)},
+        "transpose_ms": round(float(np.median([ev[0].elapsed_time(ev[1]) for ev in evs])), 4),
+    }
 
     # plain form: per (set, buffer) job one transpose and one launch per group,
     # inputs resident on the device (16-byte row stride, as the transpose needs)
@@ -286,6 +387,7 @@ def batched_measurement(args):
         "plain_launches": len(jobs) + sum(psets[si].n_kernels for si, _ in jobs),
         "launch_shape": {"groups_per_kernel": opts.get("groups_per_kernel", 1), "min_waves": opts.get("min_waves", 0)},
         "speedup": round(plain_ms / batched_ms, 3),
+        "roofline": batch_roofline,
         "form": "batched = one graph replay (1 transpose + 1 persistent launch per group); "
                 "plain = 1 transpose + 1 launch per group per job",
     }
@@ -410,6 +512,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.prepare:
+        prepare(args)
         return
     import numpy as np
     import torch

@@ -42,7 +42,12 @@ CANDIDATES = [
 ]
 
 
-def _candidates(n_groups):
+# the shape chosen for the IDS set on MI355X so far; compiled ahead of time
+# for the sets that reuse the IDS choice
+USUAL = (4, 5, 64, 1)
+
+
+def candidates(n_groups):
     """Distinct candidates for a set of n_groups groups (a merge wider than
     the set is the same kernel as merging all of it)."""
     seen, out = set(), []
@@ -76,7 +81,7 @@ def _path(key):
 
 def prepare(groups, base_options=None):
     """Compile every candidate of a set (CPU only: hipcc)."""
-    for cand in _candidates(len(groups)):
+    for cand in candidates(len(groups)):
         compiler.compile_pattern_set(groups, options_of(cand, base_options))
 
 
@@ -107,7 +112,7 @@ def tune(groups, sample, base_options=None, device=0, reps=5, use_cache=True):
     stream = torch.cuda.current_stream(dev)
     sptr = rt.current_stream_ptr(stream)
     rows = []
-    for cand in _candidates(len(groups)):
+    for cand in candidates(len(groups)):
         t0 = time.perf_counter()
         ks = compiler.compile_pattern_set(groups, options_of(cand, base_options)).load(device)
         ks.n_queues = cand[2]

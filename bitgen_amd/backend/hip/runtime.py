@@ -171,6 +171,8 @@ def transpose(bytes_ptr, n_bytes, in_stride, basis_ptr, n_words, n_inputs, strea
 class Module:
     def __init__(self, path):
         self.path = path
+        if not os.path.isfile(path):
+            raise HipRuntimeError(f"code object {path} is missing (compile the pattern set first)")
         h = ctypes.c_void_p()
         _check(load_library().bg_module_load(path.encode(), ctypes.byref(h)), f"bg_module_load({path})")
         self.handle = h

@@ -126,12 +126,29 @@ class HipGroupSet:
                 sptr = status.data_ptr() + (m * K + k) * status.size(2) * 4
                 self.launch(k, bptr, rptr, sptr, W, s)
 
-    def check_errors(self):
-        """Raise if any launch's carry look-back gave up (call after a sync)."""
-        flags = self.ctl[:, rt.CTL_ERROR].cpu()
+    def copy_errors(self, stream=None):
+        """Enqueue a copy of the kernels' error flags into pinned host memory
+        (after the launches on the same stream); ``raise_errors`` reads it once
+        the stream has synchronised, without a blocking copy of its own."""
+        torch = _torch()
+        if getattr(self, "_err_host", None) is None:
+            self._err_host = torch.zeros(self.n_kernels, dtype=torch.int32, pin_memory=True)
+        with torch.cuda.stream(stream or torch.cuda.current_stream(self.device)):
+            self._err_host.copy_(self.ctl[:, rt.CTL_ERROR], non_blocking=True)
+
+    def raise_errors(self, flags=None):
+        """Raise if a carry look-back gave up; the flags are cleared first, so
+        that later runs report only their own errors."""
+        flags = self._err_host if flags is None else flags
         if int(flags.abs().sum()) != 0:
             bad = [k for k in range(self.n_kernels) if int(flags[k]) != 0]
+            self.ctl[:, rt.CTL_ERROR].zero_()
+            flags.zero_()
             raise rt.HipRuntimeError(f"carry look-back timed out in kernels {bad}; results are invalid")
+
+    def check_errors(self):
+        """Raise if any launch's carry look-back gave up (call after a sync)."""
+        self.raise_errors(self.ctl[:, rt.CTL_ERROR].cpu())
 
     def info(self):
         return [mod.info(m["kernel"]) for mod, m in zip(self.modules, self.metas)]
@@ -160,7 +177,8 @@ def init_kernel(groups_meta, base_dir, regex_func_num, input_stream_tensor, basi
 
     def run(basic_stream_tensor):
         ks.run(basic_stream_tensor, result, status)
+        ks.copy_errors()
         rt.stream_sync(rt.current_stream_ptr())
-        ks.check_errors()
+        ks.raise_errors()
 
     return result, status, ks, run, None

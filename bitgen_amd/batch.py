@@ -245,9 +245,14 @@ class BatchPlan:
         self.graph.launch(rt.current_stream_ptr(stream))
 
     def check_errors(self):
-        for q in self.queues:
-            if int(q["ctl"][rt.CTL_ERROR].item()) != 0:
-                raise rt.HipRuntimeError("carry look-back timed out in a batched launch; results are invalid")
+        """Raise if a carry look-back of the last run gave up (call after a
+        sync); the flags are cleared first, so later replays report only
+        their own errors."""
+        bad = [i for i, q in enumerate(self.queues) if int(q["ctl"][rt.CTL_ERROR].item()) != 0]
+        if bad:
+            for i in bad:
+                self.queues[i]["ctl"][rt.CTL_ERROR].zero_()
+            raise rt.HipRuntimeError(f"carry look-back timed out in batched launches {bad}; results are invalid")
 
     def results(self, set_index, buffer_index):
         """Match streams [G, W] of one (set, buffer) job (device tensor)."""

@@ -6,6 +6,7 @@ import hashlib
 import os
 import shutil
 import subprocess
+import threading
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ROOT = os.path.dirname(PKG)
@@ -65,8 +66,10 @@ def source_key(source):
 
 
 def compile_kernel(src_path, out_path):
-    """hipcc --genco one generated .hip file into a gfx950 code object."""
-    tmp = out_path + ".tmp"
+    """hipcc --genco one generated .hip file into a gfx950 code object.  The
+    temporary name is unique per process and thread: two compiles of the same
+    source (the cache is keyed by content) never write one file."""
+    tmp = f"{out_path}.tmp{os.getpid()}_{threading.get_ident()}"
     cmd = [hipcc(), *kernel_flags(), src_path, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -113,8 +113,9 @@ def build_parser():
     return ap
 
 
-def main(argv=None):
-    args = build_parser().parse_args(argv)
+def configure(args):
+    """Set the process-global config from parsed flags (run_bitgen.py:536-586
+    of the reference)."""
     if args.backend != "hip":
         raise SystemExit(f"--backend={args.backend}: this package provides the hip backend only")
     if args.pass_graph_break != -1:
@@ -150,6 +151,11 @@ def main(argv=None):
     c("block_size", (args.cuda_block_size, 1, 1))
     c("validation", args.check)
     c("use_cached_code", args.reuse_cache)
+
+
+def main(argv=None):
+    args = build_parser().parse_args(argv)
+    configure(args)
     regex = args.regex if args.regex else args.regex_file
     try:
         global_timer.reset_timings()

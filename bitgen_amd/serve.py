@@ -68,6 +68,32 @@ def admit(waiting, in_flight_bytes, cap, max_jobs, max_bytes):
 _ZEROS = np.zeros(64, dtype=np.uint8)
 
 
+def grid_caps(occupancy, n_streams):
+    """Persistent-grid cap of every (set, kernel) launch and the sets whose
+    batches must run alone.
+
+    ``occupancy``: {(set id, kernel id): (blocks of the kernel one CU holds,
+    CUs)}.  Batches on different slots run at the same time, and a
+    persistent launch needs a resident block for every one of its tile
+    queues (a tile waits for its predecessor, which only a resident block of
+    that queue can take): launches that each hold part of the device can
+    starve each other.  A launch therefore takes at most its 1/n_streams
+    share of the blocks a CU holds of its kernel, so every slot's launch fits
+    beside the others.  A kernel that fits fewer blocks per CU than there are
+    streams has no such share: its set is exclusive -- its batches run while
+    no other batch is on the device, with the whole device as the cap.
+    Returns ({(set, kernel): max blocks}, set of exclusive set ids)."""
+    n = max(1, int(n_streams))
+    caps, exclusive = {}, set()
+    for (sid, kid), (per_cu, cus) in occupancy.items():
+        if n > 1 and per_cu < n:
+            exclusive.add(sid)
+    for (sid, kid), (per_cu, cus) in occupancy.items():
+        per_cu, cus = max(1, int(per_cu)), max(1, int(cus))
+        caps[(sid, kid)] = per_cu * cus if (sid in exclusive or n == 1) else (per_cu // n) * cus
+    return caps, exclusive
+
+
 class _Slot:
     """One stream with its in-flight batch and its own buffers, reused from
     batch to batch and grown (never shrunk) when a batch needs more: pinned
@@ -92,12 +118,25 @@ class _Slot:
         self.event = None
         self.jobs = None
         self.out_rows = None
+        self.exclusive = False
         self.t_launch = 0.0
         dtypes = {"dev_in": torch.uint8, "basis": torch.uint64, "result": torch.uint64, "status": torch.int32,
                   "host_in": torch.uint8, "host_meta": torch.uint8, "dev_meta": torch.uint8,
                   "host_out": torch.uint64}
         for name, n in self.INITIAL.items():
             self.buf(name, n, dtypes[name], pinned=name.startswith("host"))
+
+    def errors(self):
+        """Error flags of the last batch (its copy has landed once the
+        batch's event fired)."""
+        return int(self.err_host.abs().sum())
+
+    def clear_errors(self):
+        """Clear the sticky error flags of every control block of the slot
+        (ordered on the slot's stream, before its next batch)."""
+        with self.torch.cuda.stream(self.stream):
+            self.ctl[:, rt.CTL_ERROR].zero_()
+        self.err_host.zero_()
 
     def buf(self, name, n, dtype, pinned=False):
         """A buffer of at least n elements (contents undefined)."""
@@ -145,25 +184,22 @@ class MatchServer:
         self._thread = None
         props = torch.cuda.get_device_properties(self.dev)
         self.grid_waves = props.multi_processor_count * grid_waves_per_cu
-        # Batches on different slots run at the same time.  A persistent
-        # launch needs a resident block for every one of its tile queues (a
-        # tile waits for its predecessor, which only a resident block of the
-        # predecessor's queue can take), and two launches that each fill part
-        # of the device can starve each other.  So each launch takes at most
-        # its 1/n_streams share of the blocks one CU can hold of that kernel:
-        # all the slots' launches then fit on the device together.
-        self.grid_blocks_max = {}
+        occ = {}
         for sid, ps in self.sets.items():
             if ps.kernels is None:
                 raise ValueError(f"pattern set {sid} is not loaded (call BatchPatternSet.load first)")
             for kid, kern in enumerate(ps.kernels):
-                per_cu, cus = rt.kernel_occupancy(kern, ps.block_threads)
-                self.grid_blocks_max[(sid, kid)] = max(1, per_cu // max(1, n_streams)) * max(1, cus)
+                occ[(sid, kid)] = rt.kernel_occupancy(kern, ps.block_threads)
+        self.grid_blocks_max, self.exclusive_sets = grid_caps(occ, n_streams)
+        self.failed = 0
+        self.error = None  # exception that ended the scheduler thread, if any
 
     # -- client side -------------------------------------------------------------
     def submit(self, set_id, data):
         if set_id not in self.sets:
             raise KeyError(f"unknown pattern set {set_id}")
+        if self.error is not None:
+            raise RuntimeError("the match server's scheduler has stopped") from self.error
         arr = np.ascontiguousarray(np.frombuffer(data, dtype=np.uint8) if isinstance(data, (bytes, bytearray))
                                    else np.asarray(data, dtype=np.uint8))
         job = MatchJob(set_id, arr, time.perf_counter())
@@ -211,9 +247,15 @@ class MatchServer:
         self._thread.start()
         return self
 
+    def _pending(self):
+        return not self.inbox.empty() or self.waiting or any(s.jobs is not None for s in self.slots)
+
     def stop(self, drain=True):
+        """Stop serving; with drain, after every submitted job completed.
+        Returns also when the scheduler thread has ended on an error (its
+        jobs' futures then hold that error)."""
         if drain:
-            while not self.inbox.empty() or self.waiting or any(s.jobs is not None for s in self.slots):
+            while self._thread is not None and self._thread.is_alive() and self._pending():
                 time.sleep(0.001)
         self._stop.set()
         if self._thread:
@@ -223,6 +265,30 @@ class MatchServer:
 
     # -- scheduler -----------------------------------------------------------------
     def _loop(self):
+        try:
+            self._serve()
+        except BaseException as e:  # noqa: BLE001 -- every pending future gets the error
+            self.error = e
+            self._fail_all(e)
+
+    def _fail_all(self, exc):
+        """Fail every job not yet completed (in flight, waiting, in the inbox)."""
+        in_flight = [j for s in self.slots if s.jobs is not None for j in s.jobs]
+        jobs = in_flight + self.waiting
+        while True:
+            try:
+                jobs.append(self.inbox.get_nowait())
+            except queue.Empty:
+                break
+        for s in self.slots:
+            s.jobs = s.out_rows = s.event = None
+        self.waiting = []
+        self.in_flight_bytes -= sum(j.footprint for j in in_flight)
+        for job in jobs:
+            if not job.future.done():
+                job.future.set_exception(exc)
+
+    def _serve(self):
         torch = self.torch
         torch.cuda.set_device(self.dev)
         while not self._stop.is_set():
@@ -233,10 +299,17 @@ class MatchServer:
                 except queue.Empty:
                     break
             free = [s for s in self.slots if s.jobs is None]
-            if self.waiting and free:
+            exclusive_busy = any(s.jobs is not None and s.exclusive for s in self.slots)
+            if self.waiting and free and not exclusive_busy:
                 batch = self._admit()
-                if batch:
-                    self._launch(free[0], batch)
+                excl = any(j.set_id in self.exclusive_sets for j in batch)
+                if batch and excl and len(free) < len(self.slots):
+                    # an exclusive batch waits until the device is drained;
+                    # nothing else is launched meanwhile (arrival order holds)
+                    self.in_flight_bytes -= sum(j.footprint for j in batch)
+                    self.waiting = sorted(batch + self.waiting, key=lambda j: j.arrival)
+                elif batch:
+                    self._launch(free[0], batch, exclusive=excl)
                     progressed = True
             if not progressed:
                 if self.waiting or any(s.jobs is not None for s in self.slots):
@@ -252,7 +325,7 @@ class MatchServer:
             self.waiting, self.in_flight_bytes, self.cap, self.max_batch_jobs, self.max_batch_bytes)
         return batch
 
-    def _launch(self, slot, jobs):
+    def _launch(self, slot, jobs, exclusive=False):
         """Stage a batch on a free slot: one copy of all input bytes and one
         of all launch tables, one batched transpose, one persistent launch
         per (set, kernel) involved, one copy of all match streams back."""
@@ -364,6 +437,7 @@ class MatchServer:
             for i, r in zip(idx.tolist(), res_off.tolist()):
                 parts[i].append((r, n_out))
         slot.event, slot.jobs, slot.out_rows = ev, jobs, (parts, words)
+        slot.exclusive = exclusive
         slot.t_launch = time.perf_counter()
         self.batches += 1
 
@@ -372,8 +446,21 @@ class MatchServer:
         for slot in self.slots:
             if slot.jobs is None or not slot.event.query():
                 continue
-            if int(slot.err_host.abs().sum()) != 0:
-                raise rt.HipRuntimeError("carry look-back timed out in a served batch")
+            if slot.errors():
+                # a carry look-back gave up: this batch's results are
+                # invalid, the next batches are not
+                now = time.perf_counter()
+                err = rt.HipRuntimeError(f"carry look-back timed out in a served batch of {len(slot.jobs)} jobs; "
+                                         "its results are invalid")
+                for job in slot.jobs:
+                    job.done = now
+                    self.in_flight_bytes -= job.footprint
+                    job.future.set_exception(err)
+                self.failed += len(slot.jobs)
+                slot.clear_errors()
+                slot.jobs = slot.out_rows = slot.event = None
+                progressed = True
+                continue
             out = slot.bufs["host_out"].numpy()
             parts, words = slot.out_rows
             now = time.perf_counter()

@@ -175,6 +175,41 @@ def text_ids(n, seed, printable=0.7):
     return buf
 
 
+def _ids_lut():
+    """Byte for each 16-bit draw u: u < 77*256 (30.1%) gives the uniform byte
+    u & 0xFF; the other 45,824 values spread over the 95 printable bytes
+    (each 482 or 483 of them)."""
+    u = np.arange(65536)
+    lut = np.empty(65536, dtype=np.uint8)
+    wild = u < 77 * 256
+    lut[wild] = u[wild] & 0xFF
+    lut[~wild] = 0x20 + ((u[~wild] - 77 * 256) * 95) // (65536 - 77 * 256)
+    return lut
+
+
+def text_ids_fast(n, seed, chunk=1 << 24, workers=8):
+    """The ids byte mix (about 70% printable, 30% uniform bytes) from one
+    16-bit draw per position through a lookup table, in 16 MiB chunks drawn
+    by independent generators (SeedSequence children of ``seed``) on a few
+    threads: gigabytes in seconds."""
+    import concurrent.futures
+
+    lut = _ids_lut()
+    out = np.empty(n, dtype=np.uint8)
+    starts = list(range(0, n, chunk))
+    seeds = np.random.SeedSequence(seed).spawn(len(starts))
+
+    def fill(k):
+        o = starts[k]
+        m = min(chunk, n - o)
+        idx = np.frombuffer(np.random.default_rng(seeds[k]).bytes(2 * m), dtype=np.uint16)
+        out[o:o + m] = lut[idx]
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(fill, range(len(starts))))
+    return out
+
+
 def text_dna(n, seed, n_run_rate=0.001, line=80):
     """i.i.d. uniform bases, runs of N (length 10-100, starting at about
     n_run_rate/55 of the positions), LF every ``line`` bytes.  Generated in
@@ -239,22 +274,46 @@ def lognormal_sizes(n, seed, median, lo, hi, total=None, sigma=1.5):
     return draw(sigma)
 
 
+def batched_sets(set_sizes=(64, 256, 1024)):
+    """Regexes of the batched workload's pattern sets (ids mix, seeds 7, 8, 9)."""
+    return [ids_regexes(n, 7 + i) for i, n in enumerate(set_sizes)]
+
+
+SERVED_SET_SIZES = (32, 64, 128, 256, 512, 1024, 1536, 2048)
+
+
+def served_sets(set_sizes=SERVED_SET_SIZES):
+    """Regexes of the 8 resident pattern sets of the serving workload (ids
+    mix, seeds 100 + i), each balanced into groups of about 64 regexes."""
+    return [ids_regexes(n, 100 + i) for i, n in enumerate(set_sizes)]
+
+
 def batched(n_buffers=4096, total_bytes=2 * 2**30, set_sizes=(64, 256, 1024), seed=10):
     """Pattern sets of the ids mix (seeds 7, 8, 9) and buffers with log-normal
     lengths (median 16 KiB, clipped to 1 KiB-8 MiB, scaled to ~total_bytes).
     Jobs: every (set, buffer) pair."""
-    sets = [ids_regexes(n, 7 + i) for i, n in enumerate(set_sizes)]
+    sets = batched_sets(set_sizes)
     sizes = lognormal_sizes(n_buffers, seed, 16 * 1024, 1024, 8 * 2**20, total=total_bytes)
     g = np.random.default_rng(seed)
+    # one text for all buffers; buffer i is a view of its stretch
+    total = int(sizes.sum())
+    text = text_ids_fast(total, seed)
+    # planted hits (1e-4 per byte) of the regexes of set i % 3, drawn from a
+    # pool of 4,096 samples of each set's languages
+    rng = random.Random(seed)
+    pools = [[np.frombuffer(sample_match(rs[rng.randrange(len(rs))], rng), dtype=np.uint8) for _ in range(4096)]
+             for rs in sets]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
     buffers = []
-    for i, n in enumerate(sizes):
-        buf = text_ids(int(n), seed * 100003 + i)
+    for i, n in enumerate(sizes.tolist()):
+        buf = text[offs[i]:offs[i + 1]]
         k = max(1, int(n * 1e-4))
-        pos = g.integers(0, max(1, int(n) - 40), size=k)
-        rs = sets[i % len(sets)]
-        for p in pos:
-            s_ = sample_match(rs[int(g.integers(0, len(rs)))], random.Random(int(p)))
-            buf[p:p + len(s_)] = np.frombuffer(s_, dtype=np.uint8)[:max(0, int(n) - int(p))]
+        pos = g.integers(0, max(1, n - 40), size=k)
+        pick = g.integers(0, 4096, size=k)
+        pool = pools[i % len(sets)]
+        for p, s_ in zip(pos.tolist(), pick.tolist()):
+            s_ = pool[s_][:n - p]
+            buf[p:p + s_.size] = s_
         buffers.append(buf)
     jobs = [(si, b) for si in range(len(sets)) for b in range(n_buffers)]
     return sets, buffers, jobs

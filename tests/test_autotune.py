@@ -4,10 +4,10 @@ from bitgen_amd import autotune, compiler
 
 
 def test_candidates_are_distinct_and_clamped():
-    one = autotune._candidates(1)
+    one = autotune.candidates(1)
     assert len(one) == len(set(one))
     assert all(c[0] == 1 for c in one)  # a merge wider than the set is the whole set
-    many = autotune._candidates(16)
+    many = autotune.candidates(16)
     assert (4, 5, 64, 1) in many and (1, 0, 64, 0) in many
     assert len(many) == len(autotune.CANDIDATES)
 

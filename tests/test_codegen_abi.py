@@ -142,6 +142,27 @@ def test_abi_library_exports_every_declared_symbol():
     rt.load_library().bg_last_error()
 
 
+def test_missing_code_object_raises_cleanly(tmp_path):
+    """A pattern set whose .hsaco is gone fails with HipRuntimeError naming
+    the file, before any device call (the loader checks the path first)."""
+    from bitgen_amd.backend.hip import runtime as rt
+
+    missing = str(tmp_path / "kernel_0.hsaco")
+    with pytest.raises(rt.HipRuntimeError, match="kernel_0.hsaco"):
+        rt.Module(missing)
+
+
+def test_missing_library_raises_cleanly(tmp_path, monkeypatch):
+    """No libbitgen_hip.so: every entry point raises HipRuntimeError (there is
+    no fallback path)."""
+    from bitgen_amd.backend.hip import runtime as rt
+
+    monkeypatch.setattr(rt, "_lib", None)
+    monkeypatch.setenv("BITGEN_HIP_LIB", str(tmp_path / "libbitgen_hip.so"))
+    with pytest.raises(rt.HipRuntimeError, match="missing"):
+        rt.load_library()
+
+
 def test_abi_structs_and_ctl_layout_match_header(tmp_path):
     """ctypes mirrors vs the C header: sizes and offsets of every struct and
     the control-block word indices, compiled with gcc."""

@@ -44,17 +44,20 @@ def test_random_groups(gpu, seed):
     assert_parity(groups, data, gpu, repeat=2, label=f"seed {seed}")
 
 
-@pytest.mark.parametrize("seed,per_kernel", [(s, k) for s in range(6) for k in (2, 3)])
-def test_merged_group_kernels(gpu, seed, per_kernel):
+@pytest.mark.parametrize("seed,per_kernel,min_waves",
+                         [(s, k, 0) for s in range(6) for k in (2, 3)] + [(s, 4, 5) for s in range(6)])
+def test_merged_group_kernels(gpu, seed, per_kernel, min_waves):
     """Several groups merged into one kernel: every group's stream is still
-    its own, bit for bit, at every tile of long and short inputs."""
+    its own, bit for bit, at every tile of long and short inputs.  The
+    4-group cases also force 5 waves per SIMD (the register budget the
+    benchmark's shape spills under)."""
     rng = random.Random(50 + seed)
     alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
     n = rng.choice([50, 4000, 70000, 300001])
     data = np.random.default_rng(50 + seed).choice(alpha, size=n)
-    groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(2, 5))]
-    print(f"seed={seed} per_kernel={per_kernel} n={n} groups={groups}", flush=True)
-    assert_parity(groups, data, gpu, options={"groups_per_kernel": per_kernel}, repeat=2,
+    groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(2, 5) + (3 if per_kernel == 4 else 0))]
+    print(f"seed={seed} per_kernel={per_kernel} min_waves={min_waves} n={n} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, options={"groups_per_kernel": per_kernel, "min_waves": min_waves}, repeat=2,
                   label=f"merged seed {seed}")
 
 
@@ -64,8 +67,9 @@ BIGRAM_POOL = ["hello", "ab[0-9]+c", "x[yz]q", "q0q0", "[ab]cd", "zz[^\\n]*yy", 
                "dcba(xy|zq)", "c?abc", "^abc", "xyz$", "bb[0-9]{2,4}c", "\\x81\\x02", "[0-9]{2,4}"]
 
 
-@pytest.mark.parametrize("seed,per_kernel", [(s, k) for s in range(8) for k in (1, 2)])
-def test_bigram_filter_groups(gpu, seed, per_kernel):
+@pytest.mark.parametrize("seed,per_kernel,min_waves",
+                         [(s, k, 0) for s in range(8) for k in (1, 2)] + [(s, 4, 5) for s in range(8)])
+def test_bigram_filter_groups(gpu, seed, per_kernel, min_waves):
     """Per-tile byte-pair filters in front of literal first stages: tiles
     that hold a stage's pair and tiles that do not (printable text with
     planted literals, bytes above 127 whose pairs share a hash with ASCII
@@ -80,10 +84,10 @@ def test_bigram_filter_groups(gpu, seed, per_kernel):
                             dtype=np.uint8)
         off = rng.randrange(0, max(1, n - lit.size))
         data[off:off + lit.size] = lit[:n - off]
-    groups = [rng.sample(BIGRAM_POOL, rng.randint(1, 5)) for _ in range(rng.randint(1, 4))]
-    print(f"seed={seed} per_kernel={per_kernel} n={n} groups={groups}", flush=True)
-    assert_parity(groups, data, gpu, options={"bigram_filter": 1, "groups_per_kernel": per_kernel}, repeat=2,
-                  label=f"bigram seed {seed}")
+    groups = [rng.sample(BIGRAM_POOL, rng.randint(1, 5)) for _ in range(rng.randint(1, 4) + (4 if per_kernel == 4 else 0))]
+    print(f"seed={seed} per_kernel={per_kernel} min_waves={min_waves} n={n} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, options={"bigram_filter": 1, "groups_per_kernel": per_kernel,
+                                              "min_waves": min_waves}, repeat=2, label=f"bigram seed {seed}")
 
 
 @pytest.mark.parametrize("per_kernel", [1, 2])

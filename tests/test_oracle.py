@@ -229,3 +229,37 @@ def test_oracle_loop_guard_raises():
     data = np.zeros(100, dtype=np.uint8)
     with pytest.raises(RuntimeError):
         O.run(insts, vnm, O.transpose(data))
+
+
+@pytest.fixture(scope="module")
+def asan_oracle():
+    import subprocess
+
+    subprocess.run(["make", "-C", os.path.dirname(O.LIB_PATH), "oracle_asan"], check=True,
+                   stdout=subprocess.DEVNULL)
+    return os.path.join(os.path.dirname(O.LIB_PATH), "oracle_asan")
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_sanitizer_build_of_the_oracle(asan_oracle, tmp_path, seed):
+    """The oracle's C code built with AddressSanitizer and UBSan (any invalid
+    access or undefined behaviour aborts) gives the same words as the
+    library build, on random programs with loops over lengths around word
+    boundaries."""
+    import struct
+    import subprocess
+
+    rng = random.Random(1000 + seed)
+    insts, vnm = random_program(rng, n_ops=40)
+    L = rng.choice([0, 1, 63, 64, 65, 1000, 4097])
+    data = bytes(rng.randrange(256) if rng.random() < 0.5 else rng.choice(b"ab") for _ in range(L))
+    W = O.n_words_for(L)
+    recs, nslots = O.flatten(insts, vnm)
+    job = tmp_path / "job.bin"
+    job.write_bytes(struct.pack("<iiQQ", recs.shape[0], nslots, L, W) + recs.astype("<i4").tobytes() + data)
+    out = tmp_path / "out.bin"
+    r = subprocess.run([asan_oracle, str(job), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    got = np.frombuffer(out.read_bytes(), dtype=np.uint64)
+    ref = O.run_records(recs, nslots, O.transpose(np.frombuffer(data, dtype=np.uint8), W))
+    assert np.array_equal(got, ref)
