@@ -422,6 +422,13 @@ class GroupProgram:
                     bump(inst.operands[0], max(0, r[0] - n), r[1] + n)
                 else:
                     bump(inst.operands[0], r[0] + n, max(0, r[1] - n))
+            elif t in (BsInstType.MATCHSTAR, BsInstType.SCANTHRU):
+                # the wave adder works on whole words: the result is exact
+                # from lane L = HL - ceil(back / 64) on only if the operands
+                # are exact over all of lane L, i.e. from a word boundary
+                back = WAVE * math.ceil(r[0] / WAVE)
+                for o in inst.uses():
+                    bump(o, back, r[1])
             else:
                 for o in inst.uses():
                     bump(o, r[0], r[1])
