@@ -45,7 +45,12 @@ class HipGroupSet:
             path = m["hsaco"] if os.path.isabs(m["hsaco"]) else os.path.join(base_dir, m["hsaco"])
             mod = rt.Module(path)
             self.modules.append(mod)
-            self.kernels.append(mod.kernel(m["kernel"]))
+            if m.get("kind") == "stream":
+                # a group run at stream level: one kernel per step of its plan
+                self.kernels.append([mod.kernel(sm["kernel"]) for sm in m["step_kernels"]])
+            else:
+                self.kernels.append(mod.kernel(m["kernel"]))
+        self._workspaces = {}
         # first group of each kernel (metas without "groups" hold one group)
         self.first_group = []
         g = 0
@@ -76,7 +81,11 @@ class HipGroupSet:
         return 1 + (n_words - s0 + meta["S"] - 1) // meta["S"]
 
     def status_words(self, n_words):
-        return max(self.n_tiles(m, n_words) * m["status_slots"] for m in self.metas)
+        per = []
+        for m in self.metas:
+            for sm in m.get("step_kernels", [m]):
+                per.append(self.n_tiles(sm, n_words) * sm["status_slots"])
+        return max(per)
 
     def alloc_status(self, n_inputs, n_words):
         """Zeroed look-back status words [n_inputs, kernels, words] for buffers
@@ -124,7 +133,82 @@ class HipGroupSet:
             for k in range(K):
                 rptr = result.data_ptr() + (m * G + self.first_group[k]) * W * 8
                 sptr = status.data_ptr() + (m * K + k) * status.size(2) * 4
-                self.launch(k, bptr, rptr, sptr, W, s)
+                if self.metas[k].get("kind") == "stream":
+                    self._run_stream(k, basis[0, m], result[0, m, self.first_group[k]], sptr, W, stream)
+                else:
+                    self.launch(k, bptr, rptr, sptr, W, s)
+
+    # -- groups run at stream level (graph breaks, while loops) -----------------
+    # Loop guard: a while loop whose condition never empties is an error, as
+    # in the oracle (a stream of n positions cannot need more iterations for
+    # the monotone loops regexes produce; the bound is generous)
+    LOOP_GUARD_FACTOR = 4
+
+    def _workspace(self, k, W):
+        torch = _torch()
+        key = (k, W)
+        ws = self._workspaces.get(key)
+        if ws is None:
+            ws = torch.zeros((self.metas[k]["n_slots"], W), dtype=torch.uint64,
+                             device=torch.device("cuda", self.device))
+            self._workspaces[key] = ws
+        return ws
+
+    def _run_stream(self, k, basis_m, result_row, status_ptr, W, stream=None):
+        """Run group k's plan over one input: the basis is copied into
+        workspace slots 0-7, each kernel step reads workspace slots and writes
+        fresh ones, loops run while their condition slot has a set bit (one
+        host check per iteration), and the result slot is copied out.  All on
+        the given (or current) stream; the loop checks synchronise it."""
+        torch = _torch()
+        meta = self.metas[k]
+        ws = self._workspace(k, W)
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        sptr = rt.current_stream_ptr(st)
+        kernels = self.kernels[k]
+        ctl = self.ctl[k].data_ptr()
+        guard = self.LOOP_GUARD_FACTOR * 64 * W + 16
+
+        def launch(i, out_base):
+            sm = meta["step_kernels"][i]
+            args = rt.GroupArgs()
+            args.basis = ws.data_ptr()
+            args.result = ws.data_ptr() + out_base * W * 8
+            args.status = status_ptr
+            # the steps share the group's control block: each launch advances
+            # its epoch, so no step ever reads another step's status words
+            args.ctl = ctl
+            args.n_words = W
+            args.n_tiles = self.n_tiles(sm, W)
+            args.n_queues = self.n_queues
+            args.n_carry = sm["status_slots"]
+            args.take = self.take
+            rt.group_launch(kernels[i], args, sptr, self.grid_blocks)
+
+        def execute(plan):
+            for step in plan:
+                op = step[0]
+                if op == "kernel":
+                    launch(step[1], step[2])
+                elif op == "copy":
+                    ws[step[2]].copy_(ws[step[1]])
+                elif op == "zero":
+                    ws[step[1]].zero_()
+                elif op == "loop":
+                    n = 0
+                    while bool(ws[step[1]].any()):
+                        n += 1
+                        if n > guard:
+                            raise rt.HipRuntimeError(f"while loop of group {meta['gid']} did not end "
+                                                     f"after {guard} iterations")
+                        execute(step[2])
+                else:
+                    raise ValueError(f"unknown plan step {op!r}")
+
+        with torch.cuda.stream(st):
+            ws[:8].copy_(basis_m)
+            execute(meta["plan"])
+            result_row.copy_(ws[meta["result_slot"]])
 
     def copy_errors(self, stream=None):
         """Enqueue a copy of the kernels' error flags into pinned host memory
@@ -151,7 +235,12 @@ class HipGroupSet:
         self.raise_errors(self.ctl[:, rt.CTL_ERROR].cpu())
 
     def info(self):
-        return [mod.info(m["kernel"]) for mod, m in zip(self.modules, self.metas)]
+        out = []
+        for mod, m in zip(self.modules, self.metas):
+            names = [sm["kernel"] for sm in m["step_kernels"]] if m.get("kind") == "stream" else [m["kernel"]]
+            infos = [mod.info(n) for n in names]
+            out.append({f: max(i[f] for i in infos) for f in infos[0]})
+        return out
 
     def free(self):
         for mod in self.modules:

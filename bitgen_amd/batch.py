@@ -61,6 +61,9 @@ class BatchPatternSet:
             self.hsacos = [m["hsaco"] for m in cached]
             return
         programs = kernel_programs(self.groups, o)
+        if any(getattr(p, "kernel_name", "").startswith("bg_stream_") for p in programs):
+            raise NotImplementedError("groups that run at stream level (graph breaks, while loops) run in the plain "
+                                      "form only, not batched")
         self.metas = [kernel_meta(p) for p in programs]
         sources = {}
         for p in programs:

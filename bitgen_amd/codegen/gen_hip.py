@@ -160,6 +160,12 @@ class GroupProgram:
         self.min_waves = int(min_waves)
         self.kernel_name = f"bg_group_{gid}"
         self._check()
+        # streams the program loads: the 8 basis streams, or (a stream-level
+        # step) any workspace slots
+        used = sorted({int(i.stream_index) for i in insts if i.type == BsInstType.STREAMLOAD})
+        self.in_slots = list(range(8)) if not used or used[-1] < 8 else used
+        if bigram_filter and self.in_slots != list(range(8)):
+            raise HipCodegenError(f"group {gid}: byte-pair filters need the basis streams as inputs")
         self.bigram_keys = []
         self.bigram_clauses = 0
         if bigram_filter:
@@ -506,11 +512,19 @@ class GroupProgram:
             add("  bg_fence(wl);")
         else:
             add("  const uint64_t wl = (uint64_t)wc;")
-        add("  uint64_t bas[8];")
+        slots = self.in_slots
+        if slots == list(range(8)):
+            add("  uint64_t bas[8];")
+            add("#pragma unroll")
+            add("  for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + (int64_t)wl];")
+        else:
+            # a stream-level step: its inputs are workspace slots (stream s at
+            # basis + s * W)
+            add(f"  uint64_t bas[{len(slots)}];")
+            for i, sl in enumerate(slots):
+                add(f"  bas[{i}] = basis[(int64_t){sl} * W + (int64_t)wl];")
         add("#pragma unroll")
-        add("  for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + (int64_t)wl];")
-        add("#pragma unroll")
-        add("  for (int i = 0; i < 8; i++) bas[i] = inb ? bas[i] : 0ull;")
+        add(f"  for (int i = 0; i < {len(slots)}; i++) bas[i] = inb ? bas[i] : 0ull;")
         if nf:
             # byte-pair table of the tile and the answers of every filter test
             add("  bg_bigram_build(bas, bgt);")
@@ -566,7 +580,7 @@ class GroupProgram:
             for j, bits in d.items():
                 feeds.setdefault(j, []).append((k, math.ceil(bits / WAVE)))
         if speculative:
-            add("  int bg_infl = -1;" + "".join(f" int bg_bnd_{k} = -1;" for k in sorted(consumers)))
+            add("  int bg_infl = -1; uint32_t bg_vseen = 0;" + "".join(f" int bg_bnd_{k} = -1;" for k in sorted(consumers)))
         # three-input bitwise trees -> v_bitop3_b32
         trees, absorbed = self._fuse()
         self.fused = (trees, absorbed)
@@ -602,7 +616,7 @@ class GroupProgram:
                         + "".join(f" bg_bnd_{k} = -1;" for k in sorted(consumers)))
             t = inst.type
             if t == BsInstType.STREAMLOAD:
-                add(f"  {_cname(inst.ret)} = bas[{int(inst.stream_index)}];")
+                add(f"  {_cname(inst.ret)} = bas[{self.in_slots.index(int(inst.stream_index))}];")
             elif t in _BITWISE:
                 if idx not in absorbed:
                     add(f"  {_cname(inst.ret)} = {self._emit_tree(trees[idx])};")
@@ -619,6 +633,12 @@ class GroupProgram:
                     add(f"  {_cname(inst.ret)} = {fn}<{L}, S, {k}>({self.val(inst.operands[0])}, "
                         f"{self.val(inst.operands[1])}, tc, {bnd}, bg_infl);")
                     add("  if (tc.abort) goto bg_pass_end;")
+                    later = [c for c in sorted(consumers) if c > k]
+                    if k in consumers and later:
+                        # the op verified the earlier ops' assumed carry-ins:
+                        # their influence on later operands is void
+                        add("  if (tc.vfy != bg_vseen) { bg_vseen = tc.vfy;"
+                            + "".join(f" bg_bnd_{c} = -1;" for c in later) + " }")
                     for c, lanes in feeds.get(k, []):
                         add(f"  bg_bnd_{c} = bg_imax(bg_bnd_{c}, bg_infl" + (f" + {lanes});" if lanes else ");"))
                 else:

@@ -301,13 +301,33 @@ def kernel_groups(n_groups, groups_per_kernel):
 
 def kernel_programs(groups, o):
     """One GroupProgram per kernel: the kernel's groups merged into one
-    program (their streams are the program's stores, in group order)."""
+    program (their streams are the program's stores, in group order).  A
+    group whose program needs whole streams between its steps (graph breaks,
+    while loops) becomes a StreamGroup instead, and then no group of the set
+    is merged."""
+    from .codegen.gen_stream import StreamGroup, needs_stream_plan
+
+    posts = [group_programs(g, o)[2:] for g in groups]
+    if any(needs_stream_plan(insts) for insts, _ in posts):
+        programs = []
+        for g, (insts, vnm) in enumerate(posts):
+            if needs_stream_plan(insts):
+                step_opts = dict(o, bigram_filter=0)
+
+                def make(gid, step_insts, step_vnm, regs=groups[g], so=step_opts):
+                    return program_for(gid, step_insts, step_vnm, regs, so)
+
+                prog = StreamGroup(g, insts, vnm, groups[g], make)
+            else:
+                prog = program_for(g, insts, vnm, groups[g], o)
+            prog.groups = [g]
+            programs.append(prog)
+        return programs
     programs = []
     for kid, members in enumerate(kernel_groups(len(groups), o["groups_per_kernel"])):
         parts = []
         for g in members:
-            _, _, post, post_vnm = group_programs(groups[g], o)
-            parts.append((post, post_vnm))
+            parts.append(posts[g])
         insts, vnm = parts[0] if len(parts) == 1 else merge_programs(parts)
         prog = program_for(kid, insts, vnm, [r for g in members for r in groups[g]], o)
         prog.groups = members

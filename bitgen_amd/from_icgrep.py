@@ -142,10 +142,21 @@ def compile_regex(regex_str, input_path=None, input_stream_tensor=None, basic_st
     return pablo_list2kernel([[regex_str]], [pablo_path], input_path)
 
 
-def compile_xml(*args, **kwargs):
-    raise NotImplementedError(
-        "--xml: the XML lexical program's loop keeps only the last iteration's markers, which has no "
-        "tile-parallel form; the hip backend supports fixed-point star loops only")
+def compile_xml(regex_str, input_path, input_stream_tensor=None, basic_stream_tensor=None):
+    """``--xml`` (from_icgrep.py:390-425): the XML lexical program (restated
+    in tool/xml_lexical.py) compiled as one group; its pablo text is kept
+    under ``kernels/pablo/`` of the module, as the reference keeps its copy.
+    Its while loop runs at stream level (a whole-stream condition)."""
+    from .tool.xml_lexical import XML_NAME, xml_lexical_pablo
+
+    init_cached_path()
+    init_cached_module_path()
+    d = os.path.join(os.getcwd(), cfg.get_config("__cached_module_path"), "kernels", "pablo")
+    os.makedirs(d, exist_ok=True)
+    pablo_path = os.path.join(d, XML_NAME)
+    with open(pablo_path, "w") as f:
+        f.write(xml_lexical_pablo())
+    return pablo_list2kernel([regex_str], [pablo_path], input_path, input_stream_tensor, basic_stream_tensor)
 
 
 # ---------------------------------------------------------------------------

@@ -118,9 +118,6 @@ def configure(args):
     of the reference)."""
     if args.backend != "hip":
         raise SystemExit(f"--backend={args.backend}: this package provides the hip backend only")
-    if args.pass_graph_break != -1:
-        MyLogger.warning("the hip backend always fuses the whole group program (--pass-graph-break=-1)")
-        args.pass_graph_break = -1
     c = cfg.set_config
     c("warmup_iters", args.warmup_iters)
     c("exec_iters", args.exec_iters)

@@ -237,6 +237,7 @@ struct bg_tile {
   uint32_t mode;      // BG_SPECULATE or BG_REPLAY
   uint32_t k_exact;   // replay: carry-ins of ops < k_exact are in `cin`
   uint32_t abort;     // speculate: an op could not publish; the pass stops
+  uint32_t vfy;       // speculate: carry-ins of ops < vfy are known to be 0
   uint64_t cin;       // replay: carry-in bits of ops < k_exact
   uint64_t agg;       // speculate: ops that published an aggregate
   uint64_t out1;      // speculate: their carry-out for carry-in 1
@@ -258,6 +259,7 @@ __device__ __forceinline__ bg_tile bg_tile_make(uint32_t t, int64_t w0, int64_t 
   tc.mode = BG_SPECULATE;
   tc.k_exact = 0;
   tc.abort = 0;
+  tc.vfy = 0;
   tc.cin = 0;
   tc.agg = 0;
   tc.out1 = 0;
@@ -348,6 +350,33 @@ __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, u
   }
 }
 
+// Speculative pass, op K: checks that the carry-ins assumed 0 for ops
+// [tc.vfy, K) are 0, from the predecessor's statuses (one load, lane k reads
+// op k; the predecessor published these ops earlier in its own pass, so they
+// are usually there; the waiting look-back only where one is not inclusive
+// yet).  True: every op < K computed exact values, and tc.vfy = K.  False: an
+// assumption was wrong (the pass must replay).
+__device__ __forceinline__ bool bg_verify(bg_tile &tc, uint32_t K) {
+  const uint32_t ep = tc.epoch & 0x0FFFFFFFu;
+  const int lane = bg_lane();
+  const uint32_t *prev = tc.status + (size_t)(tc.t - 1) * tc.n_carry;
+  const bool mine = (uint32_t)lane >= tc.vfy && (uint32_t)lane < K;
+  uint32_t w = 0;
+  if (mine) w = bg_st_load(prev + lane);
+  const bool inc = mine && (w >> 4) == ep && ((w >> 2) & 3u) == BG_ST_INC;
+  if (__builtin_amdgcn_ballot_w64(inc && (w & 1u))) return false;
+  const uint64_t span = (K >= 64 ? ~0ull : ((1ull << K) - 1ull)) & (tc.vfy >= 64 ? 0ull : (~0ull << tc.vfy));
+  uint64_t rest = span & ~__builtin_amdgcn_ballot_w64(inc);
+  while (rest) {
+    const int k = __builtin_ctzll(rest);
+    rest &= rest - 1;
+    const uint32_t wk = (uint32_t)__builtin_amdgcn_readlane((int)w, k);
+    if (bg_lookback(tc, (uint32_t)k, wk)) return false;
+  }
+  tc.vfy = K;
+  return true;
+}
+
 // Wave-wide addition a + b of the tile's 64-word number with the carry into
 // lane L taken from the predecessor tile (decoupled look-back); returns this
 // lane's sum word.  K = index of the carry operation in the program.
@@ -367,7 +396,9 @@ __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, u
 //    advances can move values up).  The carry into lane L + S cannot depend
 //    on any assumption when a non-propagating lane lies between bnd and
 //    L + S (or bnd < L); the status is then published at once.  Otherwise
-//    the pass aborts at this op.  `infl` returns the bound for this op's
+//    the assumed carry-ins of the earlier ops are checked (bg_verify): if
+//    they hold, the operands are exact and the op publishes; if not, the
+//    pass aborts at this op.  `infl` returns the bound for this op's
 //    result.  bg_settle then resolves every carry-in with one batch of
 //    look-back loads and replays the tile if an assumption was wrong or the
 //    pass aborted.
@@ -417,12 +448,17 @@ __device__ __forceinline__ uint64_t bg_wave_add(uint64_t a, uint64_t b, bg_tile 
     }
   } else {
     // speculate: operand lanes <= b may differ from their exact values
-    const int b = bnd > 63 ? 63 : bnd;
+    int b = bnd > 63 ? 63 : bnd;
     const uint64_t notp = ~P & keep;
     if (b >= l0 && bg_first_from(notp, b + 1) >= pass) {
-      tc.abort = 1;
-      tc.k_exact = K;
-      return 0;
+      // the carry-out may depend on the carry-ins assumed for earlier ops:
+      // check them now; if they hold, the operands are exact
+      if (!bg_verify(tc, (uint32_t)K)) {
+        tc.abort = 1;
+        tc.k_exact = K;
+        return 0;
+      }
+      b = -1;
     }
     const uint32_t out0 = carry_out(0), out1 = carry_out(1);
     if (out0 == out1) {

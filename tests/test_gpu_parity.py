@@ -10,12 +10,14 @@ from tests.gpu_helpers import assert_parity
 
 pytestmark = pytest.mark.gpu
 
-# regexes whose unbounded repeats are over single classes (MatchStar form);
-# repeats of longer sub-expressions need while loops, which the hip backend
-# rejects (see test_codegen_rejects_general_loops)
-POOL = ["ab", "a[bc]+d", "x.*y", "^q", "z$", "(ab|cd){1,3}e", "b[^\\n]*c", "a(bc){0,3}d", "c?d",
-        "[0-9]{2,4}", "abcdx", "q(a|b)z*[xy]{2}", "a.{0,3}b", "(a|bb){1,4}c", "dd[^a]*c",
-        "^a[0-9]*$", "x[yz]+q*0"]
+# unbounded repeats over single classes become MatchStar (fused kernels);
+# repeats of longer sub-expressions (a(bc)*d, (ab)+c, (a|bb)+c) are while
+# loops, which run at stream level
+FUSED_POOL = ["ab", "a[bc]+d", "x.*y", "^q", "z$", "(ab|cd){1,3}e", "b[^\\n]*c", "a(bc){0,3}d", "c?d",
+              "[0-9]{2,4}", "abcdx", "q(a|b)z*[xy]{2}", "a.{0,3}b", "(a|bb){1,4}c", "dd[^a]*c",
+              "^a[0-9]*$", "x[yz]+q*0"]
+STREAM_POOL = ["a(bc)*d", "(ab)+c", "(a|bb)+c"]
+POOL = FUSED_POOL + STREAM_POOL
 
 
 @pytest.mark.parametrize("n", [0, 1, 63, 64, 65, 127, 128, 4095, 4096, 4097, 100003, 1 << 20])

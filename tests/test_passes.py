@@ -120,9 +120,32 @@ def test_remove_alias_keeps_loop_carried_and_output():
     assert any(n.startswith("test") for n in names) and any(n.startswith("accum") for n in names)
 
 
-def test_graph_break_rejects_breakdown_modes():
-    with pytest.raises(NotImplementedError):
-        graph_break_pass(0).run([], {})
+def test_graph_break_modes_place_breaks_like_the_reference():
+    """Breaks around loops/carry ops (0), also around Advances (2), before
+    every instruction but loads and stores (1); -1 leaves the program alone
+    (bitgen/passes/graph_break.py:373-396)."""
+    from bitgen_amd.inst import BsInstType
+
+    _, _, post, vnm = compiler.group_programs(["ab[0-9]{1,3}c", "x[^\\n]*yz"],
+                                              {"pass_cc_advanced": 0, "pass_short_circuit": 0})
+
+    def breaks_around(insts, kinds):
+        for k, inst in enumerate(insts):
+            if inst.type in kinds:
+                assert insts[k - 1].type == BsInstType.GRAPHBREAK and insts[k + 1].type == BsInstType.GRAPHBREAK
+
+    n = lambda insts: sum(i.type == BsInstType.GRAPHBREAK for i in insts)  # noqa: E731
+    carry = (BsInstType.MATCHSTAR, BsInstType.SCANTHRU, BsInstType.WHILE)
+    m0, _ = graph_break_pass(0).run(list(post), vnm)
+    breaks_around(m0, carry)
+    m2, _ = graph_break_pass(2).run(list(post), vnm)
+    breaks_around(m2, carry + (BsInstType.ADVANCE,))
+    m1, _ = graph_break_pass(1).run(list(post), vnm)
+    assert n(m1) == sum(i.type not in (BsInstType.STREAMLOAD, BsInstType.STREAMSTORE) for i in post)
+    assert n(m2) > n(m0) > 0
+    assert graph_break_pass(-1).run(list(post), vnm)[0] == list(post)
+    with pytest.raises(ValueError):
+        graph_break_pass(3)
 
 
 def test_group_balancing_matches_reference_rule():

@@ -36,10 +36,15 @@ def _oracle(groups, data):
 def _model(groups, data, options, basis):
     o = dict(compiler.DEFAULT_OPTIONS)
     o.update(options or {})
+    from bitgen_amd.codegen.gen_stream import StreamGroup
+    from tests.test_stream_plan import run_plan_cpu
+
     rows = []
     for prog in compiler.kernel_programs(groups, o):
-        prog.source()
-        streams = run_tiles(prog, basis)
+        if isinstance(prog, StreamGroup):
+            streams = [run_plan_cpu(prog, basis)]
+        else:
+            streams = run_tiles(prog, basis)
         rows.extend(O.mask_to_positions(s, data.size + 1) for s in streams)
     return np.stack(rows)
 

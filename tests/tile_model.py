@@ -71,8 +71,9 @@ def _wave_add(a, b, l0, cin, pass_lane):
 
 def run_tiles(prog, basis):
     """Streams [n_out, W] computed tile by tile as the generated kernel does.
-    ``prog``: a GroupProgram whose source() has been generated (so its final
-    instruction list and geometry are set); ``basis``: uint64 [8, W]."""
+    ``prog``: a GroupProgram (its final instruction list and geometry);
+    ``basis``: uint64 [streams, W] -- the 8 basis streams, or the workspace
+    of a stream-level step."""
     W = basis.shape[1]
     HL, HR, S = prog.HL, prog.HR, prog.S
     insts = prog.insts
@@ -83,7 +84,9 @@ def run_tiles(prog, basis):
     cin = [0] * prog.n_carry
     for t in range(prog.n_tiles(W)):
         w0 = 0 if t == 0 else (WAVE - HR) + (t - 1) * S - HL
-        bas = [_window(basis[i], w0, W) for i in range(8)]
+        # the streams the program loads: the basis, or workspace slots of a
+        # stream-level step (``basis`` is then the whole workspace)
+        bas = {i: _window(basis[i], w0, W) for i in prog.in_slots}
         pairs = None
         env = {}
         outs = [0] * len(stores)
@@ -127,7 +130,7 @@ def run_tiles(prog, basis):
                     env[inst.ret] = s & ~c & MASK
             elif isinstance(inst, BigramTest):
                 if pairs is None:
-                    b = _window_bytes(bas)
+                    b = _window_bytes([bas[i] for i in range(8)])
                     nxt = np.append(b[1:], 0)
                     pairs = set(((b.astype(np.int64) & 0x7F) << 8 | nxt).tolist())
                 if not all(any(h in pairs for h in clause) for clause in inst.clauses):
