@@ -49,9 +49,17 @@ def test_tile_count_covers_stream():
         assert covered >= W and (T == 1 or s0 + (T - 2) * p.S < W)
 
 
-def test_general_loops_are_rejected():
+def test_general_loops_go_to_stream_level():
+    """A fused kernel cannot hold a general while loop (its condition is a
+    whole-stream property); the set compiler runs such a group at stream
+    level instead."""
+    from bitgen_amd.codegen.gen_stream import StreamGroup
+
     with pytest.raises(HipCodegenError):
         _prog(["a(bc)*d"])
+    progs = compiler.kernel_programs([["a(bc)*d"], ["hello"]], dict(compiler.DEFAULT_OPTIONS, groups_per_kernel=2))
+    assert isinstance(progs[0], StreamGroup) and not isinstance(progs[1], StreamGroup)
+    assert "bg_group_0_s0" in progs[0].source()
 
 
 def test_generated_source_shape():
@@ -123,7 +131,8 @@ def test_generated_kernel_compiles_for_gfx950(tmp_path):
 
 
 def _declared_symbols():
-    text = open(os.path.join(ROOT, "include", "bitgen_hip.h")).read()
+    inc = os.path.join(ROOT, "include")
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     return sorted(set(re.findall(r"^\s*(?:int|uint32_t|const char \*)\s*(bg_\w+)\s*\(", text, re.M)))
 
 
@@ -337,10 +346,10 @@ def test_bitop3_fusion_computes_the_same_streams(seed):
     import numpy as np
 
     from oracle import oracle as O
-    from tests.test_gpu_parity import POOL
+    from tests.test_gpu_parity import FUSED_POOL
 
     rng = random.Random(seed)
-    regs = rng.sample(POOL, rng.randint(1, 6))
+    regs = rng.sample(FUSED_POOL, rng.randint(1, 6))
     p = _prog(regs)
     p.source()
     from bitgen_amd.codegen.gen_hip import _leaves
