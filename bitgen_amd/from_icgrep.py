@@ -59,7 +59,12 @@ def init_cached_path():
     if not cfg.get_config("__cached_path"):
         root = cfg.get_config("cache_dir", ".cache_bitgen")
         if cfg.get_config("use_cached_code"):
-            name = sorted(os.listdir(root))[-1]
+            # -r: the latest run's code (the cache root also holds code
+            # objects, indexes and tuning results)
+            runs = sorted(n for n in os.listdir(root) if n.startswith("bitgen_code_")) if os.path.isdir(root) else []
+            if not runs:
+                raise FileNotFoundError(f"-r: no cached bitgen_code_* run under {root}")
+            name = runs[-1]
         else:
             name = f"bitgen_code_{datetime.now().strftime('%Y_%m%d_%H%M%S_%f')}"
         cfg.set_config("__cached_path", os.path.join(root, name))
