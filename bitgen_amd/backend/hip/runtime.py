@@ -89,7 +89,7 @@ EXPORTS = (
     "bg_module_unload", "bg_kernel_get", "bg_kernel_info", "bg_group_launch",
     "bg_to_reference_layout", "bg_popcount", "bg_stream_sync", "bg_batch_launch",
     "bg_graph_begin", "bg_graph_end", "bg_graph_launch", "bg_graph_destroy", "bg_transpose_batch",
-    "bg_kernel_occupancy", "bg_ctl_words",
+    "bg_kernel_occupancy", "bg_ctl_words", "bg_group_launch_static",
 )
 
 _lib = None
@@ -133,6 +133,7 @@ def load_library():
         L.bg_graph_launch.argtypes = [vp, vp]
         L.bg_graph_destroy.argtypes = [vp]
         L.bg_transpose_batch.argtypes = [vp, vp, vp, u32, u64, vp]
+        L.bg_group_launch_static.argtypes = [vp, vp, vp, u64, u32, u32, vp, vp]
         for name in EXPORTS:
             if name not in ("bg_last_error", "bg_ctl_words"):
                 getattr(L, name).restype = ctypes.c_int
@@ -194,6 +195,18 @@ class Module:
         if self.handle:
             _check(load_library().bg_module_unload(self.handle), "bg_module_unload")
             self.handle = None
+
+
+# status words the static scratch of bg_group_launch_static holds
+STATIC_STATUS_WORDS = 4 << 20
+
+
+def group_launch_static(kernel, basis_ptr, result_ptr, n_words, n_tiles, n_carry, error_flag_ptr, stream):
+    """bg_group_launch_static: one group kernel over one buffer with the
+    library's static scratch (no workspace, no allocation); the launch's
+    look-back error word lands in the device u32 at error_flag_ptr."""
+    _check(load_library().bg_group_launch_static(kernel, basis_ptr, result_ptr, n_words, n_tiles, n_carry,
+                                                 error_flag_ptr, stream), "bg_group_launch_static")
 
 
 def group_launch(kernel, args, stream, grid_blocks=0):

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include "bitgen_hip.h"
+#include "bitgen_hip_static.h"
 #include "transpose_core.h"
 
 namespace {
@@ -52,6 +53,19 @@ struct bg_graph_s {
   hipGraph_t graph;
   hipGraphExec_t exec;
 };
+
+// Scratch of bg_group_launch_static: a control block (BG_CTL_WORDS words,
+// zero when the code object loads; every launch leaves its queue counters at
+// zero and advances the epoch) followed by BG_STATIC_STATUS_WORDS look-back
+// status words (epoch-tagged, never cleared).  Each device holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for that device (g_scratch).
+static __device__ __align__(256) uint32_t scratch[BG_CTL_WORDS + BG_STATIC_STATUS_WORDS];
+
+namespace {
+#define BG_MAX_DEVICES 64
+uint32_t *g_scratch[BG_MAX_DEVICES] = {};
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Transpose: thread = one 64-byte block (one word of each basis stream) of one
@@ -157,12 +171,26 @@ __global__ __launch_bounds__(256) void bg_popcount_kernel(const uint64_t *__rest
   if ((threadIdx.x & 63) == 0 && acc) atomicAdd(count, (unsigned long long)acc);
 }
 
+// Moves the sticky error word of a control block to the caller's flag and
+// clears it (one lane, vector atomics).
+__global__ void bg_take_error_kernel(uint32_t *ctl_error, uint32_t *flag) {
+  if (threadIdx.x != 0) return;
+  const uint32_t e = __hip_atomic_load(ctl_error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(flag, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(ctl_error, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
 int bg_init(int device) {
   BG_HIP(hipSetDevice(device), "hipSetDevice");
   BG_HIP(hipFree(nullptr), "runtime init");
+  if (device >= 0 && device < BG_MAX_DEVICES && !g_scratch[device]) {
+    void *p = nullptr;
+    BG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = static_cast<uint32_t *>(p);
+  }
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
     g_cu_count = cus;
@@ -330,6 +358,37 @@ int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t grid_bl
   void *params[] = {&a};
   BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, threads, 1, 1, 0, S(stream), params, nullptr),
          "hipModuleLaunchKernel");
+  return 0;
+}
+
+int bg_group_launch_static(bg_kernel_t k, const uint64_t *basis, uint64_t *result, uint64_t n_words,
+                           uint32_t n_tiles, uint32_t n_carry, uint32_t *error_flag, void *stream) {
+  int dev = -1;
+  BG_HIP(hipGetDevice(&dev), "hipGetDevice");
+  uint32_t *s = (dev >= 0 && dev < BG_MAX_DEVICES) ? g_scratch[dev] : nullptr;
+  if (!s) return set_err(BG_EINVAL, "bg_group_launch_static: call bg_init for this device first");
+  if (!k || !basis || !result || !error_flag) return set_err(BG_EINVAL, "bg_group_launch_static: null argument");
+  if (n_carry == 0) return set_err(BG_EINVAL, "bg_group_launch_static: n_carry must be at least 1");
+  if ((uint64_t)n_tiles * n_carry > BG_STATIC_STATUS_WORDS) {
+    snprintf(g_err, sizeof(g_err),
+             "bg_group_launch_static: %llu status words needed (%u tiles x %u), the static scratch holds %u",
+             (unsigned long long)n_tiles * n_carry, n_tiles, n_carry, (unsigned)BG_STATIC_STATUS_WORDS);
+    return BG_EINVAL;
+  }
+  bg_group_args_t a;
+  a.basis = basis;
+  a.result = result;
+  a.status = s + BG_CTL_WORDS;
+  a.ctl = s;
+  a.n_words = n_words;
+  a.n_tiles = n_tiles;
+  a.n_queues = 0;
+  a.n_carry = n_carry;
+  a.take = 0;
+  int rc = bg_group_launch(k, &a, 0, stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(bg_take_error_kernel, dim3(1), dim3(64), 0, S(stream), s + BG_CTL_ERROR, error_flag);
+  BG_HIP(hipGetLastError(), "bg_group_launch_static error hand-off");
   return 0;
 }
 
