@@ -108,12 +108,16 @@ def prepare(args):
     t0 = time.perf_counter()
     ids_groups = compiler.group_regex_in_list(workloads.ids_regexes(args.regexes, 3), args.groups)
     gen_groups = compiler.group_regex_in_list(workloads.genomic_regexes(512, 5), 32)
-    opts = autotune.options_of(autotune.USUAL)
     tasks = [(compiler.compile_pattern_set, g, autotune.options_of(c))
              for g in (ids_groups, gen_groups) for c in autotune.candidates(len(g))]
+    # the batched and served sets take the IDS set's tuned shape: prepare the
+    # usual one at every block size the tuner may pick
+    usual = autotune.USUAL
+    shapes = [autotune.options_of(usual[:4] + (bt,)) for bt in sorted({c[4] for c in autotune.CANDIDATES})]
     for regs in workloads.batched_sets() + workloads.served_sets():
         g = compiler.group_regex_in_list(regs, workloads.groups_for(len(regs)))
-        tasks += [(compiler.compile_pattern_set, g, opts), (BatchPatternSet, g, opts)]
+        for opts in shapes:
+            tasks += [(compiler.compile_pattern_set, g, opts), (BatchPatternSet, g, opts)]
     with concurrent.futures.ThreadPoolExecutor(max_workers=3) as ex:
         for f in [ex.submit(fn, g, o) for fn, g, o in tasks]:
             f.result()
@@ -244,8 +248,8 @@ def genomic_measurement(args, steps=3):
     out = {"workload": "genomic", "regexes": len(regexes), "groups": ks.n_groups, "kernels": ks.n_kernels,
            "text_bytes": n_bytes, "ms_per_pass": round(ms, 3), "gbs": round(n_bytes / (ms / 1e3) / 1e9, 3),
            "options": options or "default (one kernel per group)", "steps": steps,
-           "autotune": [(c["groups_per_kernel"], c["min_waves"], c.get("bigram_filter", 0), c["ms"])
-                        for c in tuning["candidates"]] if tuning else None}
+           "autotune": [(c["groups_per_kernel"], c["min_waves"], c.get("bigram_filter", 0), c.get("block_threads", 256),
+                         c["ms"]) for c in tuning["candidates"]] if tuning else None}
     _log(f"genomic: {out}")
     ks.free()
     del basis, staged, result, status
@@ -270,7 +274,10 @@ def batched_measurement(args):
          f"{[len(s) for s in sets]} regexes in {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
     dev = torch.device("cuda", 0)
     opts = dict(getattr(args, "tuned_options", {}) or {})
-    bsets = [BatchPatternSet(g, opts).load(0) for g in set_groups]
+    bsets = []
+    for g in set_groups:
+        bsets.append(BatchPatternSet(g, opts).load(0))
+        _log(f"batched: set of {len(g)} groups loaded ({time.perf_counter() - t0:.1f}s)")
     plan = BatchPlan(bsets, buffers, jobs)
     scanned = plan.scanned_bytes()
     stream = torch.cuda.current_stream()
@@ -327,9 +334,11 @@ def batched_measurement(args):
 
     # plain form: per (set, buffer) job one transpose and one launch per group,
     # inputs resident on the device (16-byte row stride, as the transpose needs)
-    psets = [compiler.compile_pattern_set(g, opts).load(0) for g in set_groups]
-    for ks in psets:
-        ks.n_queues = args.queues
+    psets = []
+    for g in set_groups:
+        psets.append(compiler.compile_pattern_set(g, opts).load(0))
+        psets[-1].n_queues = args.queues
+    _log("batched: plain form loaded")
     sptr = rt.current_stream_ptr(stream)
     staged, basis, results, status = [], [], {}, {}
     for b in buffers:
@@ -385,7 +394,8 @@ def batched_measurement(args):
         "plain_ms": round(plain_ms, 4),
         "plain_gbs": round(scanned / (plain_ms / 1e3) / 1e9, 3),
         "plain_launches": len(jobs) + sum(psets[si].n_kernels for si, _ in jobs),
-        "launch_shape": {"groups_per_kernel": opts.get("groups_per_kernel", 1), "min_waves": opts.get("min_waves", 0)},
+        "launch_shape": {k: opts.get(k, compiler.DEFAULT_OPTIONS[k])
+                         for k in ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads")},
         "speedup": round(plain_ms / batched_ms, 3),
         "roofline": batch_roofline,
         "form": "batched = one graph replay (1 transpose + 1 persistent launch per group); "
@@ -410,8 +420,11 @@ def served_measurement(args):
     set_groups = [compiler.group_regex_in_list(workloads.ids_regexes(n, 100 + i), workloads.groups_for(n))
                   for i, n in enumerate(sizes)]
     opts = dict(getattr(args, "tuned_options", {}) or {})
-    bsets = {i: BatchPatternSet(g, opts).load(0) for i, g in enumerate(set_groups)}
-    psets = {i: compiler.compile_pattern_set(g, opts).load(0) for i, g in enumerate(set_groups)}
+    bsets, psets = {}, {}
+    for i, g in enumerate(set_groups):
+        bsets[i] = BatchPatternSet(g, opts).load(0)
+        psets[i] = compiler.compile_pattern_set(g, opts).load(0)
+        _log(f"served: set {i} ({len(g)} groups) loaded ({time.perf_counter() - t0:.1f}s)")
     _log(f"served pattern sets loaded: {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
     dev = torch.device("cuda", 0)
     out = {"workload": "served", "sets_regexes": list(sizes), "sets_groups": [len(g) for g in set_groups],
@@ -542,7 +555,7 @@ def main():
         if not args.queues:
             args.queues = tuning["queues"]
         _log(f"autotuner: {tuning['options']} queues {tuning['queues']} "
-             f"({[(c['groups_per_kernel'], c['min_waves'], c.get('bigram_filter', 0), c['ms']) for c in tuning['candidates']]})")
+             f"({[(c['groups_per_kernel'], c['min_waves'], c.get('bigram_filter', 0), c.get('block_threads', 256), c['ms']) for c in tuning['candidates']]})")
     args.tuned_options = options
     cs = compiler.compile_pattern_set(groups, options)
     print(f"[bench] compiled {len(groups)} groups into {len(cs.metas)} kernels ({time.perf_counter() - t0:.1f}s)",

@@ -14,7 +14,9 @@ onto gfx950:
 * ``queues`` -- tile-queue counters the launch spreads its tile takes over;
 * ``bigram_filter`` -- per-tile byte-pair filters in front of the literal
   first stages of the regexes (a tile whose bytes hold none of a stage's
-  pairs skips that regex without evaluating its character classes).
+  pairs skips that regex without evaluating its character classes);
+* ``block_threads`` -- waves per block (x 64 threads): the bytes a block
+  scans per round of takes, and the granularity the CUs are filled in.
 
 The result is cached under ``cache_dir/tune/<set digest>.json`` together
 with every candidate's time, keyed by the pattern set, the sample size, the
@@ -30,29 +32,35 @@ import time
 from . import config as cfg
 from . import compiler
 
-# candidate launch shapes: (groups_per_kernel, min_waves, queues, bigram_filter)
+# candidate launch shapes: (groups_per_kernel, min_waves, queues,
+# bigram_filter, block_threads).  A tile is one wave whatever the block size,
+# so block_threads sets the waves per block (and the bytes a block scans per
+# round: 4 KiB per wave); words per lane stay 1 (a 64-bit word per lane is the
+# wave64 shift and carry unit).
 CANDIDATES = [
-    (1, 0, 64, 0),
-    (2, 5, 64, 0),
-    (4, 5, 64, 0),
-    (4, 0, 64, 0),
-    (8, 5, 64, 0),
-    (1, 0, 64, 1),
-    (4, 5, 64, 1),
+    (1, 0, 64, 0, 256),
+    (2, 5, 64, 0, 256),
+    (4, 5, 64, 0, 256),
+    (4, 0, 64, 0, 256),
+    (8, 5, 64, 0, 256),
+    (1, 0, 64, 1, 256),
+    (4, 5, 64, 1, 256),
+    (4, 5, 64, 1, 128),
+    (4, 5, 64, 1, 512),
 ]
 
 
 # the shape chosen for the IDS set on MI355X so far; compiled ahead of time
 # for the sets that reuse the IDS choice
-USUAL = (4, 5, 64, 1)
+USUAL = (4, 5, 64, 1, 256)
 
 
 def candidates(n_groups):
     """Distinct candidates for a set of n_groups groups (a merge wider than
     the set is the same kernel as merging all of it)."""
     seen, out = set(), []
-    for gpk, mw, q, bg in CANDIDATES:
-        cand = (max(1, min(gpk, n_groups)), mw, q, bg)
+    for gpk, mw, q, bg, bt in CANDIDATES:
+        cand = (max(1, min(gpk, n_groups)), mw, q, bg, bt)
         if cand not in seen:
             seen.add(cand)
             out.append(cand)
@@ -60,9 +68,9 @@ def candidates(n_groups):
 
 
 def options_of(candidate, base=None):
-    gpk, mw, _q, bg = candidate
+    gpk, mw, _q, bg, bt = candidate
     o = dict(base or {})
-    o.update(groups_per_kernel=gpk, min_waves=mw, bigram_filter=bg)
+    o.update(groups_per_kernel=gpk, min_waves=mw, bigram_filter=bg, block_threads=bt)
     return o
 
 
@@ -136,7 +144,7 @@ def tune(groups, sample, base_options=None, device=0, reps=5, use_cache=True):
         ms = e0.elapsed_time(e1) / reps
         info = ks.info()
         rows.append({"groups_per_kernel": cand[0], "min_waves": cand[1], "queues": cand[2],
-                     "bigram_filter": cand[3], "ms": round(ms, 4),
+                     "bigram_filter": cand[3], "block_threads": cand[4], "ms": round(ms, 4),
                      "kernels": ks.n_kernels, "vgprs": max(i["vgprs"] for i in info),
                      "scratch_bytes": max(i["scratch_bytes"] for i in info),
                      "setup_s": round(time.perf_counter() - t0, 2)})
@@ -144,7 +152,7 @@ def tune(groups, sample, base_options=None, device=0, reps=5, use_cache=True):
     best = min(rows, key=lambda r: r["ms"])
     out = {
         "options": options_of((best["groups_per_kernel"], best["min_waves"], best["queues"],
-                                best["bigram_filter"]), base_options),
+                                best["bigram_filter"], best["block_threads"]), base_options),
         "queues": best["queues"],
         "ms": best["ms"],
         "sample_bytes": int(data.size),

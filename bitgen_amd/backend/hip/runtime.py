@@ -89,8 +89,35 @@ EXPORTS = (
     "bg_module_unload", "bg_kernel_get", "bg_kernel_info", "bg_group_launch",
     "bg_to_reference_layout", "bg_popcount", "bg_stream_sync", "bg_batch_launch",
     "bg_graph_begin", "bg_graph_end", "bg_graph_launch", "bg_graph_destroy", "bg_transpose_batch",
-    "bg_kernel_occupancy", "bg_ctl_words", "bg_group_launch_static",
+    "bg_kernel_occupancy", "bg_ctl_words", "bg_group_launch_static", "bg_serve_pack", "bg_serve_enqueue",
 )
+
+ENOSPACE = 1003
+
+
+class ServeKernel(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_void_p), ("hr", ctypes.c_uint32), ("s", ctypes.c_uint32),
+                ("status_slots", ctypes.c_uint32), ("n_out", ctypes.c_uint32), ("block_threads", ctypes.c_uint32),
+                ("grid_cap", ctypes.c_uint32)]
+
+
+class ServeSet(ctypes.Structure):
+    _fields_ = [("first_kernel", ctypes.c_uint32), ("n_kernels", ctypes.c_uint32)]
+
+
+class ServeLayout(ctypes.Structure):
+    _fields_ = [("stage_bytes", ctypes.c_uint64), ("basis_words", ctypes.c_uint64),
+                ("result_words", ctypes.c_uint64), ("status_words", ctypes.c_uint64),
+                ("tdesc_off", ctypes.c_uint64), ("n_jobs", ctypes.c_uint32), ("n_queues", ctypes.c_uint32)]
+
+
+class ServeQueue(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_uint32), ("n_jobs", ctypes.c_uint32), ("total_tiles", ctypes.c_uint32),
+                ("blocks", ctypes.c_uint32), ("jobs_off", ctypes.c_uint64), ("tile_job_off", ctypes.c_uint64)]
+
+
+# bg_serve_job_t as a numpy record: the job list is built vectorised
+SERVE_JOB_DTYPE = [("data", "<u8"), ("n_bytes", "<u8"), ("set", "<u4"), ("reserved", "<u4")]
 
 _lib = None
 _lock = threading.Lock()
@@ -134,6 +161,9 @@ def load_library():
         L.bg_graph_destroy.argtypes = [vp]
         L.bg_transpose_batch.argtypes = [vp, vp, vp, u32, u64, vp]
         L.bg_group_launch_static.argtypes = [vp, vp, vp, u64, u32, u32, vp, vp]
+        L.bg_serve_pack.argtypes = [vp, u32, vp, u32, vp, u32, vp, u64, u64, u64, u64, u64, u64, u64,
+                                    ctypes.POINTER(ServeLayout), vp, u32, vp, u32]
+        L.bg_serve_enqueue.argtypes = [ctypes.POINTER(ServeLayout), vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         for name in EXPORTS:
             if name not in ("bg_last_error", "bg_ctl_words"):
                 getattr(L, name).restype = ctypes.c_int

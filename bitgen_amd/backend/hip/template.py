@@ -196,7 +196,8 @@ class HipGroupSet:
                     ws[step[1]].zero_()
                 elif op == "loop":
                     n = 0
-                    while bool(ws[step[1]].any()):
+                    # (torch has no any() for uint64 on the device: view as int64)
+                    while bool(ws[step[1]].view(torch.int64).any()):
                         n += 1
                         if n > guard:
                             raise rt.HipRuntimeError(f"while loop of group {meta['gid']} did not end "

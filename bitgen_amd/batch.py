@@ -51,6 +51,8 @@ class BatchPatternSet:
         o.update(options or {})
         self.groups = [list(g) for g in groups]
         self.options = o
+        # the set's block size (compile option) unless the caller names one
+        block_threads = int(o.get("block_threads") or block_threads)
         self.block_threads = block_threads
         self.modules = None
         self.kernels = None

@@ -127,7 +127,7 @@ class GroupProgram:
     """Analysis + HIP source of one group program."""
 
     def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
-                 min_waves=0, bigram_filter=False, replay_scope="stage"):
+                 min_waves=0, bigram_filter=False, replay_scope="stage", block_threads=256):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -144,7 +144,13 @@ class GroupProgram:
         pass recomputes -- "stage": from the first zero test or carry op on
         (the basis and the class header stay in registers across passes),
         "tile": the whole tile from its basis loads (nothing stays live
-        across the pass loop's back edge but the tile's carry state)."""
+        across the pass loop's back edge but the tile's carry state).
+        block_threads: threads per block of the plain kernel (waves per
+        block x 64; a wave is a tile either way, the runtime launches the
+        kernel with the block size its launch bounds name)."""
+        if block_threads % WAVE or not 64 <= block_threads <= 1024:
+            raise ValueError(f"block_threads must be a multiple of 64 in 64..1024, not {block_threads}")
+        self.block_threads = int(block_threads)
         if replay_scope not in ("stage", "tile"):
             raise ValueError(f"replay_scope must be 'stage' or 'tile', not {replay_scope!r}")
         self.replay_scope = replay_scope
@@ -938,11 +944,11 @@ class GroupProgram:
         return "\n".join([
             '#include "bitgen_device.h"',
             self.device_function(),
-            f'extern "C" __global__ {self.launch_bounds()} void {self.kernel_name}(bg_group_args_t a) {{',
+            f'extern "C" __global__ {self.launch_bounds(self.block_threads)} void {self.kernel_name}(bg_group_args_t a) {{',
             "  // persistent: each wave takes tiles from its block's queue until the",
             "  // queue is exhausted",
             "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles, a.take);",
-            *self.table_decl(256),
+            *self.table_decl(self.block_threads),
             "  uint32_t first;",
             "  bool more = bg_queue_take(Q, first);",
             "  while (more) {",
@@ -1040,12 +1046,20 @@ class HipGenerator(Generator):
         self.programs = []
 
     def build_programs(self):
+        """One program per group: a fused GroupProgram, or a StreamGroup when
+        the group needs whole streams between its steps (graph breaks,
+        while loops)."""
+        from .gen_stream import StreamGroup, needs_stream_plan
+
         self.programs = []
         for g, (insts, vnm) in enumerate(zip(self.insts_list, self.var_name_map_list)):
             regs = self.regex_list[g] if self.regex_list else None
             if isinstance(regs, str):
                 regs = [regs]
-            self.programs.append(GroupProgram(g, insts, vnm, regs))
+            if needs_stream_plan(insts):
+                self.programs.append(StreamGroup(g, insts, vnm, regs, lambda gid, i, v, r=regs: GroupProgram(gid, i, v, r)))
+            else:
+                self.programs.append(GroupProgram(g, insts, vnm, regs))
         return self.programs
 
     def lower(self, input_path=None):

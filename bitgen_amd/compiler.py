@@ -47,6 +47,8 @@ DEFAULT_OPTIONS = {
     # answer literal first-stage zero tests from a per-tile byte-pair table
     "bigram_filter": 0,
     "replay_scope": "stage",
+    # threads per block of the plain kernels (4 waves of one tile each)
+    "block_threads": 256,
 }
 
 
@@ -202,7 +204,7 @@ def program_for(gid, insts, vnm, regexes, options):
     return GroupProgram(gid, insts, vnm, regexes, lookback=options["lookback"],
                         expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]),
                         min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]),
-                        replay_scope=options["replay_scope"])
+                        replay_scope=options["replay_scope"], block_threads=int(options.get("block_threads", 256)))
 
 
 @dataclass

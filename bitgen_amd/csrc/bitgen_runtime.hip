@@ -12,6 +12,7 @@
 
 #include "bitgen_hip.h"
 #include "bitgen_hip_static.h"
+#include "bitgen_hip_serve.h"
 #include "transpose_core.h"
 
 namespace {
@@ -47,6 +48,7 @@ struct bg_kernel_s {
   hipFunction_t fn;
   uint32_t occ_threads;  // block size of the cached occupancy (0 = none)
   int32_t occ_blocks;    // blocks per CU at that block size
+  uint32_t threads;      // block size of a plain launch: the kernel's launch bounds (0 = not read yet)
 };
 
 struct bg_graph_s {
@@ -263,7 +265,7 @@ int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out) {
     snprintf(g_err, sizeof(g_err), "kernel %s not found: %s", name, hipGetErrorString(e));
     return BG_ENOTFOUND;
   }
-  *out = new bg_kernel_s{f};
+  *out = new bg_kernel_s{f, 0, 0, 0};
   return 0;
 }
 
@@ -348,7 +350,14 @@ int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t grid_bl
   if (!args->basis || !args->result || !args->status || !args->ctl)
     return set_err(BG_EINVAL, "bg_group_launch: null buffer");
   if (args->n_tiles == 0) return 0;
-  const uint32_t threads = 256;
+  if (k->threads == 0) {
+    // the block size the kernel was compiled for (its launch bounds): one
+    // wave per tile, as many waves per block as the generator chose
+    int v = 0;
+    BG_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn), "max threads");
+    k->threads = (v >= 64 && v <= 1024 && v % 64 == 0) ? (uint32_t)v : 256u;
+  }
+  const uint32_t threads = k->threads;
   uint32_t blocks = 0;
   int rc = persistent_grid(k, threads, args->n_tiles, clamp_take(args->take), grid_blocks, &blocks);
   if (rc) return rc;
@@ -475,3 +484,5 @@ int bg_stream_sync(void *stream) {
 }
 
 }  // extern "C"
+
+#include "bitgen_serve.inc"

@@ -12,6 +12,7 @@ so it does not depend on which other jobs shared the batch.
 
 The reference has no serving component; this is the north star's addition.
 """
+import ctypes
 import gc
 import queue
 import threading
@@ -23,12 +24,6 @@ import numpy as np
 
 from .backend.hip import runtime as rt
 from .batch import words_for
-
-_JOB_DTYPE = np.dtype([("basis", "<u8"), ("result", "<u8"), ("status", "<u8"), ("n_words", "<u8"),
-                       ("group", "<u4"), ("n_carry", "<u4"), ("tile_start", "<u4"), ("n_tiles", "<u4")])
-_TDESC_DTYPE = np.dtype([("byte_off", "<u8"), ("n_bytes", "<u8"), ("basis_off", "<u8"), ("n_words", "<u8"),
-                         ("word_start", "<u8")])
-
 
 @dataclass
 class MatchJob:
@@ -65,9 +60,6 @@ def admit(waiting, in_flight_bytes, cap, max_jobs, max_bytes):
     return batch, rest, in_flight_bytes
 
 
-_ZEROS = np.zeros(64, dtype=np.uint8)
-
-
 def grid_caps(occupancy, n_streams):
     """Persistent-grid cap of every (set, kernel) launch and the sets whose
     batches must run alone.
@@ -96,55 +88,61 @@ def grid_caps(occupancy, n_streams):
 
 class _Slot:
     """One stream with its in-flight batch and its own buffers, reused from
-    batch to batch and grown (never shrunk) when a batch needs more: pinned
-    host staging for the input bytes and the launch tables, device buffers
-    for input, basis, match streams, look-back status and tables, a pinned
-    host buffer for the match streams, and one control block per (set,
-    kernel) with a pinned copy of their error flags.  A slot's buffers are
-    touched only while it holds no batch, so reuse never races the GPU."""
+    batch to batch and grown (never shrunk) when a batch needs more: a
+    pinned staging buffer the native runtime packs the batch into and its
+    device copy, device buffers for basis, match streams and look-back
+    status, a pinned buffer the match streams (and the batch's error flag)
+    come back to, and one control block that every launch of the slot uses
+    (so the epochs of its status words never repeat: the status buffer is
+    zeroed only when it is allocated).  A slot's buffers are touched only
+    while it holds no batch, so reuse never races the GPU."""
 
     # initial capacities (elements): enough for a batch of a few MiB, so
     # steady serving never allocates (pinned allocations take milliseconds)
-    INITIAL = {"dev_in": 8 << 20, "basis": 1 << 20, "result": 4 << 20, "status": 1 << 20,
-               "host_in": 8 << 20, "host_meta": 1 << 20, "dev_meta": 1 << 20, "host_out": 4 << 20}
+    INITIAL = {"stage": 16 << 20, "dev_stage": 16 << 20, "basis": 1 << 20, "result": 4 << 20,
+               "status": 1 << 20, "host_out": 4 << 20}
+    DTYPES = {"stage": "uint8", "dev_stage": "uint8", "basis": "uint64", "result": "uint64", "status": "int32",
+              "host_out": "uint64"}
 
-    def __init__(self, torch, dev, n_ctl):
+    def __init__(self, torch, dev, max_queues, max_jobs, max_kernels):
         self.torch = torch
         self.dev = dev
         self.stream = torch.cuda.Stream(device=dev)
-        self.ctl = torch.zeros((max(1, n_ctl), rt.CTL_WORDS), dtype=torch.int32, device=dev)
-        self.err_host = torch.zeros(max(1, n_ctl), dtype=torch.int32, pin_memory=True)
+        self.stream_ptr = rt.current_stream_ptr(self.stream)
+        self.ctl = torch.zeros(rt.CTL_WORDS, dtype=torch.int32, device=dev)
+        self.flag = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.queues = (rt.ServeQueue * max_queues)()
+        self.job_result = np.zeros(max_jobs * max_kernels, dtype=np.uint64)
+        self.layout = rt.ServeLayout()
         self.bufs = {}
         self.event = None
         self.jobs = None
         self.out_rows = None
         self.exclusive = False
         self.t_launch = 0.0
-        dtypes = {"dev_in": torch.uint8, "basis": torch.uint64, "result": torch.uint64, "status": torch.int32,
-                  "host_in": torch.uint8, "host_meta": torch.uint8, "dev_meta": torch.uint8,
-                  "host_out": torch.uint64}
         for name, n in self.INITIAL.items():
-            self.buf(name, n, dtypes[name], pinned=name.startswith("host"))
+            self.buf(name, n)
 
     def errors(self):
-        """Error flags of the last batch (its copy has landed once the
-        batch's event fired)."""
-        return int(self.err_host.abs().sum())
+        """Error flag of the last batch (its copy has landed once the batch's
+        event fired)."""
+        return int(self.bufs["host_out"][self.layout.result_words].item() & 0xFFFFFFFF)
 
     def clear_errors(self):
-        """Clear the sticky error flags of every control block of the slot
-        (ordered on the slot's stream, before its next batch)."""
-        with self.torch.cuda.stream(self.stream):
-            self.ctl[:, rt.CTL_ERROR].zero_()
-        self.err_host.zero_()
+        """Nothing to clear: the batch's launches moved the control block's
+        error word into the batch's own flag."""
 
-    def buf(self, name, n, dtype, pinned=False):
-        """A buffer of at least n elements (contents undefined)."""
+    def buf(self, name, n):
+        """A buffer of at least n elements (contents undefined, but a new
+        status buffer is zeroed)."""
         t = self.bufs.get(name)
         if t is None or t.numel() < n:
             size = max(int(n), int(1.5 * t.numel()) if t is not None else 0, 1024)
-            if pinned:
+            dtype = getattr(self.torch, self.DTYPES[name])
+            if name in ("stage", "host_out"):
                 t = self.torch.empty(size, dtype=dtype, pin_memory=True)
+            elif name == "status":
+                t = self.torch.zeros(size, dtype=dtype, device=self.dev)
             else:
                 t = self.torch.empty(size, dtype=dtype, device=self.dev)
             self.bufs[name] = t
@@ -170,12 +168,6 @@ class MatchServer:
         self.inbox = queue.Queue()
         self.waiting = []
         self.in_flight_bytes = 0
-        # control-block row of every (set, kernel) pair, the same in each slot
-        self.ctl_row = {}
-        for sid, ps in self.sets.items():
-            for kid in range(len(ps.metas)):
-                self.ctl_row[(sid, kid)] = len(self.ctl_row)
-        self.slots = [_Slot(torch, self.dev, len(self.ctl_row)) for _ in range(n_streams)]
         self.completed = []
         self.batches = 0
         # per completed batch: (jobs, host staging seconds, launch-to-done seconds)
@@ -191,6 +183,25 @@ class MatchServer:
             for kid, kern in enumerate(ps.kernels):
                 occ[(sid, kid)] = rt.kernel_occupancy(kern, ps.block_threads)
         self.grid_blocks_max, self.exclusive_sets = grid_caps(occ, n_streams)
+        # native tables of the resident kernels: one entry per (set, kernel)
+        self.set_index = {sid: i for i, sid in enumerate(self.sets)}
+        n_k = sum(len(ps.metas) for ps in self.sets.values())
+        self.c_kernels = (rt.ServeKernel * max(1, n_k))()
+        self.c_sets = (rt.ServeSet * max(1, len(self.sets)))()
+        self.kernel_n_out = []
+        k = 0
+        for sid, ps in self.sets.items():
+            self.c_sets[self.set_index[sid]] = rt.ServeSet(k, len(ps.metas))
+            wpb = ps.block_threads // 64
+            for kid, meta in enumerate(ps.metas):
+                cap = max(1, min(self.grid_waves // wpb, self.grid_blocks_max[(sid, kid)]))
+                self.c_kernels[k] = rt.ServeKernel(ps.kernels[kid], meta["HR"], meta["S"], meta["status_slots"],
+                                                   meta.get("n_out", 1), ps.block_threads, cap)
+                self.kernel_n_out.append(meta.get("n_out", 1))
+                k += 1
+        self.max_kernels = max(len(ps.metas) for ps in self.sets.values())
+        self.slots = [_Slot(torch, self.dev, max(1, n_k), max_batch_jobs + len(self.sets), self.max_kernels)
+                      for _ in range(n_streams)]
         self.failed = 0
         self.error = None  # exception that ended the scheduler thread, if any
 
@@ -326,117 +337,50 @@ class MatchServer:
         return batch
 
     def _launch(self, slot, jobs, exclusive=False):
-        """Stage a batch on a free slot: one copy of all input bytes and one
-        of all launch tables, one batched transpose, one persistent launch
-        per (set, kernel) involved, one copy of all match streams back."""
-        torch = self.torch
+        """Stage a batch on a free slot in the native runtime: bg_serve_pack
+        lays out input bytes and launch tables in the slot's pinned buffer,
+        bg_serve_enqueue copies them over, transposes, launches every (set,
+        kernel) of the batch and copies the match streams back, all on the
+        slot's stream."""
         slot.t_start = time.perf_counter()
         n = len(jobs)
-        lengths = np.array([j.data.size for j in jobs], dtype=np.int64)
-        words = (lengths + 1 + 63) // 64
-        padded = (lengths + 63) // 64 * 64
-        byte_off = np.concatenate([[0], np.cumsum(padded)])[:-1]
-        total_bytes = int(padded.sum()) or 64
-        basis_off = np.concatenate([[0], np.cumsum(8 * words)])[:-1]
-        # per (set, kernel) queues over this batch's jobs of that set; a job's
-        # result holds its kernel's n_out group streams in order.  Everything
-        # per job is vectorised: a batch can hold hundreds of jobs.
-        queues, res_total, stat_total = [], 0, 0
-        set_ids = np.array([j.set_id for j in jobs])
-        for sid in dict.fromkeys(set_ids.tolist()):
-            ps = self.sets[sid]
-            idx = np.nonzero(set_ids == sid)[0]
-            Wj = words[idx]
-            for kid, meta in enumerate(ps.metas):
-                C = meta["status_slots"]
-                n_out = meta.get("n_out", 1)
-                first = meta.get("first_group", kid)
-                s0 = 64 - meta["HR"]
-                T = np.where(Wj <= s0, 1, 1 + (Wj - s0 + meta["S"] - 1) // meta["S"]).astype(np.int64)
-                ent = np.zeros(len(idx), dtype=_JOB_DTYPE)
-                ent["n_words"] = Wj
-                ent["group"] = first
-                ent["n_carry"] = C
-                ent["tile_start"] = np.concatenate([[0], np.cumsum(T)])[:-1]
-                ent["n_tiles"] = T
-                res_off = res_total + np.concatenate([[0], np.cumsum(Wj * n_out)])[:-1]
-                stat_off = stat_total + np.concatenate([[0], np.cumsum(T * C)])[:-1]
-                res_total += int(Wj.sum()) * n_out
-                stat_total += int((T * C).sum())
-                queues.append((sid, kid, idx, ent, res_off, stat_off, int(T.sum()), T, first, n_out))
-        res_total, stat_total = max(res_total, 1), max(stat_total, 1)
-        # device buffers of this slot (grown if needed), then the tables that
-        # point into them, packed into one byte image with 256-byte sections
-        dbytes = slot.buf("dev_in", total_bytes, torch.uint8)
-        basis = slot.buf("basis", int(8 * words.sum()) or 1, torch.uint64)
-        result = slot.buf("result", res_total, torch.uint64)
-        status = slot.buf("status", stat_total, torch.int32)
-        td = np.zeros(n, dtype=_TDESC_DTYPE)
-        td["byte_off"] = byte_off
-        td["n_bytes"] = lengths
-        td["basis_off"] = basis_off
-        td["n_words"] = words
-        td["word_start"] = np.concatenate([[0], np.cumsum(words)])[:-1]
-        sections = [td.view(np.uint8)]
-        for (sid, kid, idx, ent, res_off, stat_off, total, T, first, n_out) in queues:
-            ent["basis"] = basis.data_ptr() + 8 * basis_off[idx]
-            ent["result"] = result.data_ptr() + 8 * res_off
-            ent["status"] = status.data_ptr() + 4 * stat_off
-            sections.append(ent.view(np.uint8))
-            sections.append(np.repeat(np.arange(len(idx), dtype=np.int32), T).view(np.uint8))
-        offs, o = [], 0
-        for sec in sections:
-            offs.append(o)
-            o += (sec.size + 255) // 256 * 256
-        meta_bytes = max(o, 256)
-        host_in = slot.buf("host_in", total_bytes, torch.uint8, pinned=True)
-        hv = host_in.numpy()
-        # one concatenation into the pinned buffer: each job's bytes, then
-        # zeros up to its 64-byte boundary
-        pad = padded - lengths
-        pieces = []
-        for j, p in zip(jobs, pad.tolist()):
-            pieces.append(j.data)
-            if p:
-                pieces.append(_ZEROS[:p])
-        if pieces:
-            np.concatenate(pieces, out=hv[:int(padded.sum())])
-        host_meta = slot.buf("host_meta", meta_bytes, torch.uint8, pinned=True)
-        mv = host_meta.numpy()
-        for sec, off in zip(sections, offs):
-            mv[off:off + sec.size] = sec
-        dmeta = slot.buf("dev_meta", meta_bytes, torch.uint8)
-        host_out = slot.buf("host_out", res_total, torch.uint64, pinned=True)
-        with torch.cuda.stream(slot.stream):
-            dbytes[:total_bytes].copy_(host_in[:total_bytes], non_blocking=True)
-            dmeta[:meta_bytes].copy_(host_meta[:meta_bytes], non_blocking=True)
-            # status words must start clear: regions move between (set,
-            # kernel) pairs from batch to batch, and their epochs differ
-            status[:stat_total].zero_()
-            s = rt.current_stream_ptr(slot.stream)
-            rt.transpose_batch(dbytes.data_ptr(), basis.data_ptr(), dmeta.data_ptr() + offs[0], n, int(words.sum()), s)
-            for q, (sid, kid, idx, ent, res_off, stat_off, total, T, first, n_out) in enumerate(queues):
-                ps = self.sets[sid]
-                args = rt.BatchArgs()
-                args.jobs = dmeta.data_ptr() + offs[1 + 2 * q]
-                args.tile_job = dmeta.data_ptr() + offs[2 + 2 * q]
-                args.ctl = slot.ctl[self.ctl_row[(sid, kid)]].data_ptr()
-                args.n_jobs = len(idx)
-                args.total_tiles = total
-                wpb = ps.block_threads // 64
-                blocks = max(1, min(self.grid_waves // wpb, self.grid_blocks_max[(sid, kid)],
-                                    (total + wpb - 1) // wpb))
-                rt.batch_launch(ps.kernels[kid], args, blocks, ps.block_threads, s)
-            host_out[:res_total].copy_(result[:res_total], non_blocking=True)
-            slot.err_host.copy_(slot.ctl[:, rt.CTL_ERROR], non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(slot.stream)
-        # per job: (result offset, first group, n_out) of each of its kernels
-        parts = [[] for _ in jobs]
-        for (sid, kid, idx, ent, res_off, stat_off, total, T, first, n_out) in queues:
-            for i, r in zip(idx.tolist(), res_off.tolist()):
-                parts[i].append((r, n_out))
-        slot.event, slot.jobs, slot.out_rows = ev, jobs, (parts, words)
+        L = rt.load_library()
+        jt = np.empty(n, dtype=rt.SERVE_JOB_DTYPE)
+        jt["data"] = [j.data.ctypes.data for j in jobs]
+        jt["n_bytes"] = [j.data.size for j in jobs]
+        jt["set"] = [self.set_index[j.set_id] for j in jobs]
+        jt["reserved"] = 0
+        if slot.job_result.size < n * self.max_kernels:
+            slot.job_result = np.zeros(n * self.max_kernels, dtype=np.uint64)
+        lay = slot.layout
+        while True:
+            b = slot.bufs
+            rc = L.bg_serve_pack(self.c_sets, len(self.sets), self.c_kernels, len(self.kernel_n_out),
+                                 jt.ctypes.data, n, b["stage"].data_ptr(), b["stage"].numel(),
+                                 b["basis"].data_ptr(), b["basis"].numel(), b["result"].data_ptr(),
+                                 b["result"].numel(), b["status"].data_ptr(), b["status"].numel(),
+                                 ctypes.byref(lay), slot.queues, len(slot.queues), slot.job_result.ctypes.data,
+                                 self.max_kernels)
+            if rc != rt.ENOSPACE:
+                rt._check(rc, "bg_serve_pack")
+                break
+            slot.buf("stage", lay.stage_bytes)
+            slot.buf("dev_stage", lay.stage_bytes)
+            slot.buf("basis", lay.basis_words)
+            slot.buf("result", lay.result_words)
+            slot.buf("status", lay.status_words)
+        slot.buf("dev_stage", lay.stage_bytes)
+        host_out = slot.buf("host_out", lay.result_words + 1)
+        rt._check(L.bg_serve_enqueue(ctypes.byref(lay), slot.queues, self.c_kernels, slot.bufs["stage"].data_ptr(),
+                                     slot.bufs["dev_stage"].data_ptr(), slot.bufs["basis"].data_ptr(),
+                                     slot.bufs["result"].data_ptr(), slot.ctl.data_ptr(), slot.flag.data_ptr(),
+                                     host_out.data_ptr(), slot.stream_ptr), "bg_serve_enqueue")
+        ev = self.torch.cuda.Event()
+        ev.record(slot.stream)
+        # per job: words and the result offset of each of its set's kernels
+        words = (jt["n_bytes"] + 1 + 63) // 64
+        slot.event, slot.jobs = ev, jobs
+        slot.out_rows = (slot.job_result[:n * self.max_kernels].copy(), words)
         slot.exclusive = exclusive
         slot.t_launch = time.perf_counter()
         self.batches += 1
@@ -462,13 +406,19 @@ class MatchServer:
                 progressed = True
                 continue
             out = slot.bufs["host_out"].numpy()
-            parts, words = slot.out_rows
+            offs, words = slot.out_rows
             now = time.perf_counter()
             self.batch_log.append((len(slot.jobs), slot.t_launch - slot.t_start, now - slot.t_launch))
+            mk = self.max_kernels
             for i, job in enumerate(slot.jobs):
                 W = int(words[i])
-                # kernels were queued in group order, so the blocks are too
-                res = np.concatenate([out[r:r + n * W].reshape(n, W) for r, n in parts[i]])
+                cs = self.c_sets[self.set_index[job.set_id]]
+                # the set's kernels in order: their groups are consecutive
+                parts = []
+                for kk in range(cs.n_kernels):
+                    r, nk = int(offs[i * mk + kk]), self.kernel_n_out[cs.first_kernel + kk]
+                    parts.append(out[r:r + nk * W].reshape(nk, W))
+                res = parts[0].copy() if len(parts) == 1 else np.concatenate(parts)
                 job.done = now
                 self.in_flight_bytes -= job.footprint
                 self.completed.append(job)

@@ -8,14 +8,16 @@ def test_candidates_are_distinct_and_clamped():
     assert len(one) == len(set(one))
     assert all(c[0] == 1 for c in one)  # a merge wider than the set is the whole set
     many = autotune.candidates(16)
-    assert (4, 5, 64, 1) in many and (1, 0, 64, 0) in many
+    assert (4, 5, 64, 1, 256) in many and (1, 0, 64, 0, 256) in many
+    assert {c[4] for c in many} == {128, 256, 512}
     assert len(many) == len(autotune.CANDIDATES)
 
 
 def test_options_of_candidate_keep_the_base():
-    o = autotune.options_of((8, 5, 64, 1), {"lookback": "speculate", "min_waves": 2})
-    assert o == {"lookback": "speculate", "groups_per_kernel": 8, "min_waves": 5, "bigram_filter": 1}
-    for key in ("groups_per_kernel", "min_waves", "bigram_filter"):
+    o = autotune.options_of((8, 5, 64, 1, 512), {"lookback": "speculate", "min_waves": 2})
+    assert o == {"lookback": "speculate", "groups_per_kernel": 8, "min_waves": 5, "bigram_filter": 1,
+                 "block_threads": 512}
+    for key in ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads"):
         assert key in compiler.DEFAULT_OPTIONS
 
 

@@ -92,7 +92,7 @@ def ids_sample(ids_workload):
     return groups, sample, oracle_streams(groups, sample)
 
 
-@pytest.mark.parametrize("cand", autotune.candidates(16), ids=lambda c: "gpk%d_mw%d_q%d_bg%d" % c)
+@pytest.mark.parametrize("cand", autotune.candidates(16), ids=lambda c: "gpk%d_mw%d_q%d_bg%d_bt%d" % c)
 def test_ids_every_tuner_candidate(gpu, ids_sample, cand):
     groups, sample, ref = ids_sample
     print(f"IDS 16 MiB candidate {cand}", flush=True)
@@ -107,7 +107,7 @@ def genomic_sample():
     return groups, buf, oracle_streams(groups, buf)
 
 
-@pytest.mark.parametrize("cand", autotune.candidates(32), ids=lambda c: "gpk%d_mw%d_q%d_bg%d" % c)
+@pytest.mark.parametrize("cand", autotune.candidates(32), ids=lambda c: "gpk%d_mw%d_q%d_bg%d_bt%d" % c)
 def test_genomic_every_tuner_candidate(gpu, genomic_sample, cand):
     groups, buf, ref = genomic_sample
     print(f"genomic 2 MiB candidate {cand}", flush=True)

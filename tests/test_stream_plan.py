@@ -103,3 +103,22 @@ def test_fused_groups_stay_fused():
     assert not needs_stream_plan(post)
     _, _, post2, _ = compiler.group_programs(["a(bc)*d"])
     assert needs_stream_plan(post2)
+
+
+def test_generator_plugin_routes_loops_to_stream_level():
+    """The reference-facing generator (create_generator("hip")) builds a
+    StreamGroup for a group with a general loop (the --xml program among
+    them) and a fused GroupProgram otherwise."""
+    from bitgen_amd.codegen import create_generator
+    from bitgen_amd.codegen.gen_hip import GroupProgram
+    from bitgen_amd.tool.xml_lexical import xml_lexical_pablo
+
+    xml, xvnm = compiler.parse_pablo(xml_lexical_pablo())
+    xml, xvnm = compiler.apply_passes(xml, xvnm, {})
+    _, _, plain, pvnm = compiler.group_programs(["hello", "x.*y"])
+    gen = create_generator("hip")([xml, plain], [xvnm, pvnm], regex_list=[["xml_parser"], ["hello", "x.*y"]])
+    progs = gen.build_programs()
+    assert isinstance(progs[0], StreamGroup) and isinstance(progs[1], GroupProgram)
+    m = progs[0].meta()
+    assert m["kind"] == "stream" and m["plan"][1][0] == "loop"
+    assert all(f"bg_group_0_s{i}" in progs[0].source() for i in range(len(m["step_kernels"])))

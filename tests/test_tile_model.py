@@ -131,7 +131,7 @@ def ids_prefix():
     return groups, data
 
 
-@pytest.mark.parametrize("cand", autotune.candidates(16), ids=lambda c: "gpk%d_mw%d_q%d_bg%d" % c)
+@pytest.mark.parametrize("cand", autotune.candidates(16), ids=lambda c: "gpk%d_mw%d_q%d_bg%d_bt%d" % c)
 def test_ids_prefix_every_tuner_shape(ids_prefix, cand):
     groups, data = ids_prefix
     ref = _assert_model(groups, data, autotune.options_of(cand), label=f"IDS 1 MiB {cand}")
