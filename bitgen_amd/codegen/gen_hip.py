@@ -127,7 +127,7 @@ class GroupProgram:
     """Analysis + HIP source of one group program."""
 
     def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
-                 min_waves=0, bigram_filter=False, replay_scope="stage", block_threads=256):
+                 min_waves=0, bigram_filter=False, replay_scope="stage", block_threads=256, carry_settle="inline"):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -147,7 +147,12 @@ class GroupProgram:
         across the pass loop's back edge but the tile's carry state).
         block_threads: threads per block of the plain kernel (waves per
         block x 64; a wave is a tile either way, the runtime launches the
-        kernel with the block size its launch bounds name)."""
+        kernel with the block size its launch bounds name).  carry_settle
+        (speculative programs): "inline" -- each tile settles its carry-ins
+        at the end of its pass, waiting for its predecessor, and replays in
+        a loop; "launch" -- the kernel runs twice, a sweep that never waits
+        and a settle that replays only the tiles that need it (one pass per
+        tile and launch, so no pass loop)."""
         if block_threads % WAVE or not 64 <= block_threads <= 1024:
             raise ValueError(f"block_threads must be a multiple of 64 in 64..1024, not {block_threads}")
         self.block_threads = int(block_threads)
@@ -156,6 +161,9 @@ class GroupProgram:
         self.replay_scope = replay_scope
         if lookback not in ("speculate", "wait"):
             raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
+        if carry_settle not in ("inline", "launch"):
+            raise ValueError(f"carry_settle must be 'inline' or 'launch', not {carry_settle!r}")
+        self.carry_settle = carry_settle
         self.gid = gid
         self.insts = insts
         self.vnm = vnm
@@ -461,6 +469,11 @@ class GroupProgram:
                 self.carry_ops.append((inst, L))
         self.n_carry = len(self.carry_ops)
 
+    @property
+    def two_launch(self):
+        """The kernel runs as a sweep launch and a settle launch."""
+        return 0 < self.n_carry <= 64 and self.lookback == "speculate" and self.carry_settle == "launch"
+
     def n_tiles(self, n_words):
         s0 = WAVE - self.HR
         if n_words <= s0:
@@ -489,20 +502,28 @@ class GroupProgram:
             add(f"static __constant__ uint32_t bg_keys_{self.gid}[{len(words)}] = {{{keys}}};")
         add(f"static __device__ __forceinline__ void bg_prog_{self.gid}(const uint64_t *__restrict__ basis, "
             "uint64_t *__restrict__ result, uint32_t *status, int64_t W, uint32_t t, uint32_t epoch, "
-            "uint32_t n_carry, uint32_t *err, uint32_t *bgt) {")
+            "uint32_t n_carry, uint32_t *err, uint32_t *bgt, uint32_t phase) {")
         add(f"  constexpr int HL = {self.HL}, HR = {self.HR}, S = {self.S};")
         add("  const int lane = bg_lane();")
         add("  const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;")
         add("  const int64_t w = w0 + lane;")
         add("  const bool inb = (w >= 0) && (w < W);")
         add("  bg_tile tc = bg_tile_make(t, w0, W, status, n_carry, epoch, err);")
+        two = self.two_launch
+        if two:
+            # settle launch: most tiles are exact from the sweep and stop here,
+            # before any load
+            add("  tc.sweep = phase == 0;")
+            add(f"  if (phase != 0 && bg_settle_begin<{self.n_carry}>(tc)) return;")
+        else:
+            add("  (void)phase;")
         if self.n_carry > 64 or self.lookback == "wait":
             # every op waits for its predecessor (speculation state is kept in
             # 64-bit masks, so programs with more carry ops always wait)
             add("  tc.mode = BG_REPLAY;")
         add("  (void)tc; (void)bgt;")
         outs = [f"out{j}" for j in range(self.n_out)]
-        tile_scope = 0 < self.n_carry <= 64 and self.lookback == "speculate" and self.replay_scope == "tile"
+        tile_scope = 0 < self.n_carry <= 64 and self.lookback == "speculate" and self.replay_scope == "tile" and not two
         # all eight loads in flight at once: clamp the index instead of
         # predicating each load, then clear the lanes outside the stream
         add("  const int64_t wc = w < 0 ? 0 : (w >= W ? W - 1 : w);")
@@ -563,6 +584,8 @@ class GroupProgram:
         # reset at the top of a pass, so a replay computes exactly what a
         # single straight run would, and the header stays outside.
         speculative = 0 < self.n_carry <= 64 and self.lookback == "speculate"
+        # the two-launch form runs one pass per tile and launch: no pass loop
+        looped = speculative and not two
         body_start = next((k for k, i in enumerate(self.insts)
                            if i.type in (BsInstType.SKIPZERO, BsInstType.MATCHSTAR, BsInstType.SCANTHRU)),
                           len(self.insts))
@@ -575,7 +598,7 @@ class GroupProgram:
         body_vars = list(dict.fromkeys(d for i in self.insts[body_start:] for d in i.defs()))
         if tile_scope:
             body_start = 0
-        self.loop_start = body_start if speculative else None
+        self.loop_start = body_start if looped else None
         self.carry_bounds = self._carry_bounds(carry_index)
         # operand bound of carry op k = max over the carry ops j it depends on
         # of (j's result bound + lanes of right advance from j to k), kept as
@@ -609,7 +632,7 @@ class GroupProgram:
         for idx, inst in enumerate(self.insts):
             lines = chunks[idx]
             add = lines.append
-            if speculative and idx == body_start:
+            if looped and idx == body_start:
                 if not tile_scope:
                     add("  for (;;) {")
                 if self.bigram_keys:
@@ -694,10 +717,13 @@ class GroupProgram:
                 lines.extend(head)
             elif idx not in hoisted:
                 lines.extend(chunks[idx])
-        if speculative:
+        if looped:
             add("bg_pass_end:;")
             add(f"  if (bg_settle<{self.n_carry}>(tc)) break;")
             add("  }")
+        elif two:
+            add("bg_pass_end:;")
+            add(f"  if (!bg_sweep_end<{self.n_carry}>(tc)) return;")
         add("  const int lo = (t == 0) ? 0 : HL;")
         add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
         add("  if (inb && lane >= lo && lane < hi) {")
@@ -944,7 +970,8 @@ class GroupProgram:
         return "\n".join([
             '#include "bitgen_device.h"',
             self.device_function(),
-            f'extern "C" __global__ {self.launch_bounds(self.block_threads)} void {self.kernel_name}(bg_group_args_t a) {{',
+            f'extern "C" __global__ {self.launch_bounds(self.block_threads)} void {self.kernel_name}(bg_group_args_t a, '
+            'uint32_t phase) {',
             "  // persistent: each wave takes tiles from its block's queue until the",
             "  // queue is exhausted",
             "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles, a.take);",
@@ -955,13 +982,25 @@ class GroupProgram:
             "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
             "    for (uint32_t i = 0, t = first; i < Q.take && t < Q.n; i++, t += Q.nq)",
             f"      bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
-            "a.ctl + BG_CTL_ERROR, bgt);",
+            "a.ctl + BG_CTL_ERROR, bgt, phase);",
             "    more = bg_queue_finish(Q, next, first);",
             "  }",
-            "  bg_queue_close(Q);",
+            self.close_line(),
             "}",
+            *self.two_launch_marker(self.kernel_name),
             "",
         ])
+
+    def close_line(self):
+        # a sweep keeps the epoch: its settle launch reads the same statuses
+        return "  bg_queue_close(Q, phase != 0);" if self.two_launch else "  bg_queue_close(Q);"
+
+    def two_launch_marker(self, kernel):
+        """An empty kernel named <kernel>_two_launch: its presence in the code
+        object tells the runtime to launch `kernel` twice (phase 0, then 1)."""
+        if not self.two_launch:
+            return []
+        return [f'extern "C" __global__ void {kernel}_two_launch() {{}}']
 
     def meta(self):
         return {
@@ -973,6 +1012,8 @@ class GroupProgram:
             "n_carry": self.n_carry,
             # look-back status words per tile: one per carry op (>= 1)
             "status_slots": max(1, self.n_carry),
+            # launches per run: 2 for the sweep + settle form
+            "launches": 2 if self.two_launch else 1,
             "n_out": self.n_out,
             "n_bigram_tests": len(self.bigram_keys),
             "n_insts": len(self.insts),
@@ -993,7 +1034,8 @@ def batch_source(program, block_threads=256):
     return "\n".join([
         '#include "bitgen_device.h"',
         program.device_function(),
-        f'extern "C" __global__ {program.launch_bounds(block_threads)} void {batch_kernel_name(g)}(bg_batch_args_t a) {{',
+        f'extern "C" __global__ {program.launch_bounds(block_threads)} void {batch_kernel_name(g)}(bg_batch_args_t a, '
+        'uint32_t phase) {',
         "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles, a.take);",
         *program.table_decl(block_threads),
         "  uint32_t first;",
@@ -1004,12 +1046,13 @@ def batch_source(program, block_threads=256):
         "      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
         "      const bg_job_t J = a.jobs[j];",
         f"      bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, Q.epoch, J.n_carry, "
-        "a.ctl + BG_CTL_ERROR, bgt);",
+        "a.ctl + BG_CTL_ERROR, bgt, phase);",
         "    }",
         "    more = bg_queue_finish(Q, next, first);",
         "  }",
-        "  bg_queue_close(Q);",
+        program.close_line(),
         "}",
+        *program.two_launch_marker(batch_kernel_name(g)),
         "",
     ])
 

@@ -49,6 +49,11 @@ DEFAULT_OPTIONS = {
     "replay_scope": "stage",
     # threads per block of the plain kernels (4 waves of one tile each)
     "block_threads": 256,
+    # speculative carry-ins settled "inline" (each tile at the end of its
+    # pass, waiting for its predecessor) or by a second "launch" of the
+    # kernel (a sweep that never waits, then a settle that replays only the
+    # tiles that need it)
+    "carry_settle": "inline",
 }
 
 
@@ -204,7 +209,8 @@ def program_for(gid, insts, vnm, regexes, options):
     return GroupProgram(gid, insts, vnm, regexes, lookback=options["lookback"],
                         expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]),
                         min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]),
-                        replay_scope=options["replay_scope"], block_threads=int(options.get("block_threads", 256)))
+                        replay_scope=options["replay_scope"], block_threads=int(options.get("block_threads", 256)),
+                        carry_settle=options.get("carry_settle", "inline"))
 
 
 @dataclass

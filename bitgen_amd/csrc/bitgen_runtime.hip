@@ -49,6 +49,7 @@ struct bg_kernel_s {
   uint32_t occ_threads;  // block size of the cached occupancy (0 = none)
   int32_t occ_blocks;    // blocks per CU at that block size
   uint32_t threads;      // block size of a plain launch: the kernel's launch bounds (0 = not read yet)
+  uint32_t launches;     // 2: a sweep and a settle launch per run (phase argument 0, then 1)
 };
 
 struct bg_graph_s {
@@ -265,7 +266,16 @@ int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out) {
     snprintf(g_err, sizeof(g_err), "kernel %s not found: %s", name, hipGetErrorString(e));
     return BG_ENOTFOUND;
   }
-  *out = new bg_kernel_s{f, 0, 0, 0};
+  // a generated kernel in the sweep + settle form comes with an empty
+  // marker kernel <name>_two_launch
+  uint32_t launches = 1;
+  char marker[512];
+  if (snprintf(marker, sizeof(marker), "%s_two_launch", name) < (int)sizeof(marker)) {
+    hipFunction_t m;
+    if (hipModuleGetFunction(&m, module->mod, marker) == hipSuccess) launches = 2;
+    (void)hipGetLastError();  // a missing marker is not an error
+  }
+  *out = new bg_kernel_s{f, 0, 0, 0, launches};
   return 0;
 }
 
@@ -364,9 +374,11 @@ int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t grid_bl
   bg_group_args_t a = *args;
   a.n_queues = clamp_queues(a.n_queues, blocks);
   a.take = clamp_take(a.take);
-  void *params[] = {&a};
-  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, threads, 1, 1, 0, S(stream), params, nullptr),
-         "hipModuleLaunchKernel");
+  for (uint32_t phase = 0; phase < k->launches; phase++) {
+    void *params[] = {&a, &phase};
+    BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, threads, 1, 1, 0, S(stream), params, nullptr),
+           "hipModuleLaunchKernel");
+  }
   return 0;
 }
 
@@ -437,9 +449,11 @@ int bg_batch_launch(bg_kernel_t k, const bg_batch_args_t *args, uint32_t grid_bl
   bg_batch_args_t a = *args;
   a.n_queues = clamp_queues(a.n_queues, blocks);
   a.take = clamp_take(a.take);
-  void *params[] = {&a};
-  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, block_threads, 1, 1, 0, S(stream), params, nullptr),
-         "hipModuleLaunchKernel(batch)");
+  for (uint32_t phase = 0; phase < k->launches; phase++) {
+    void *params[] = {&a, &phase};
+    BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, block_threads, 1, 1, 0, S(stream), params, nullptr),
+           "hipModuleLaunchKernel(batch)");
+  }
   return 0;
 }
 

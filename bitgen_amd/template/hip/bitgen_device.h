@@ -234,6 +234,7 @@ struct bg_tile {
   uint32_t epoch;     // launch epoch (status words of older launches are ignored)
   uint32_t *error;    // set to nonzero when a look-back wait gives up
   // Carry-in handling, wave-uniform (see bg_wave_add and bg_settle).
+  uint32_t sweep;     // 1: first launch of a two-launch kernel (bg_sweep_end)
   uint32_t mode;      // BG_SPECULATE or BG_REPLAY
   uint32_t k_exact;   // replay: carry-ins of ops < k_exact are in `cin`
   uint32_t abort;     // speculate: an op could not publish; the pass stops
@@ -256,6 +257,7 @@ __device__ __forceinline__ bg_tile bg_tile_make(uint32_t t, int64_t w0, int64_t 
   tc.n_carry = n_carry;
   tc.epoch = epoch;
   tc.error = error;
+  tc.sweep = 0;
   tc.mode = BG_SPECULATE;
   tc.k_exact = 0;
   tc.abort = 0;
@@ -275,6 +277,9 @@ __device__ __forceinline__ bg_tile bg_tile_make(uint32_t t, int64_t w0, int64_t 
 // Status word: bits 0..1 value, bits 2..3 kind, bits 4..31 epoch.
 #define BG_ST_AGG 1u   // aggregate of a propagating segment (value unused)
 #define BG_ST_INC 2u   // inclusive carry into the successor (value = carry)
+#define BG_ST_DEF 3u   // deferred: the tile's sweep stopped before this op; its
+                       // settle launch publishes the status
+#define BG_LB_UNKNOWN 2u  // bg_lookback in a sweep: a deferred status decides
 
 __device__ __forceinline__ uint32_t bg_st_pack(uint32_t epoch, uint32_t kind, uint32_t v) {
   return (epoch << 4) | (kind << 2) | v;
@@ -309,6 +314,9 @@ __device__ __forceinline__ int bg_first_from(uint64_t notp, int x) {
 // themselves), which shortens the walks of later tiles.  st is tile t - 1's
 // status word if already loaded (else 0).  Waits are bounded so that a
 // missing predecessor shows up as an error flag instead of a hung wave.
+// A deferred status is waited for like a missing one, except in a sweep
+// launch, where only the next launch publishes it: the result is then
+// BG_LB_UNKNOWN.
 __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, uint32_t st) {
   const uint32_t ep = tc.epoch & 0x0FFFFFFFu;
   int64_t j = (int64_t)tc.t - 1;
@@ -320,7 +328,8 @@ __device__ __forceinline__ uint32_t bg_lookback(const bg_tile &tc, uint32_t K, u
     if (!have) st = (uint32_t)__builtin_amdgcn_readfirstlane((int)bg_st_load(tc.status + (size_t)j * tc.n_carry + K));
     have = false;
     const uint32_t kind = (st >> 2) & 3u;
-    if ((st >> 4) != ep || kind == 0) {
+    if ((st >> 4) == ep && kind == BG_ST_DEF && tc.sweep) return BG_LB_UNKNOWN;
+    if ((st >> 4) != ep || kind == 0 || kind == BG_ST_DEF) {
       if (++spins > BG_SPIN_LIMIT) {
         if (bg_lane() == 0) __hip_atomic_fetch_or(tc.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return 0;
@@ -535,6 +544,75 @@ __device__ __forceinline__ bool bg_settle(bg_tile &tc) {
   return false;
 }
 
+// ---- two-launch form ---------------------------------------------------------
+// The same kernel runs twice over the same tiles and epoch.  The sweep
+// (launch 1) runs every tile's speculative pass once and never waits at its
+// end: it stores the tile's values, which are exact when every assumed
+// carry-in turns out 0, and leaves the carry-ins to the settle (launch 2).
+// By then every tile has published every op's status, or marked the ops
+// from its abort point on as deferred, so the settle reads each tile's
+// carry-ins with no wait except behind deferred tiles, and replays only
+// tiles whose assumption was wrong or whose sweep aborted.  A tile runs one
+// pass per launch: nothing stays live across passes.
+
+// End of a sweep pass.  True: the pass reached its end and its values are
+// stored (exact if every carry-in is 0; otherwise the settle replays the
+// tile and stores again).  False: it aborted at op k_exact; ops k_exact..NC-1
+// are marked deferred (one store per lane) and nothing is stored.
+template <int NC>
+__device__ __forceinline__ bool bg_sweep_end(const bg_tile &tc) {
+  static_assert(NC >= 1 && NC <= 64, "the two-launch form needs 1..64 carry operations");
+  if (!tc.abort) return true;
+  const uint32_t lane = (uint32_t)bg_lane();
+  if (lane >= tc.k_exact && lane < (uint32_t)NC)
+    bg_st_store(tc.status + (size_t)tc.t * tc.n_carry + lane, bg_st_pack(tc.epoch, BG_ST_DEF, 0u));
+  return false;
+}
+
+// Start of a tile in the settle launch.  Reads the tile's own statuses and
+// its predecessor's (one load each, lane k = op k): ops before the first
+// deferred op k_exact take their carry-ins from the predecessor (the
+// waiting look-back only where the predecessor's status is an aggregate or
+// deferred), and ops that published an aggregate publish their inclusive
+// value (an aggregate's carry-out is its carry-in).  True: the sweep's values
+// are exact, the tile is done.  False: the tile replays (mode BG_REPLAY with
+// the resolved carry-ins of ops < k_exact; later ops wait for theirs).
+template <int NC>
+__device__ __forceinline__ bool bg_settle_begin(bg_tile &tc) {
+  static_assert(NC >= 1 && NC <= 64, "the two-launch form needs 1..64 carry operations");
+  if (tc.t == 0) return true;  // tile 0 is exact from its sweep
+  const uint32_t ep = tc.epoch & 0x0FFFFFFFu;
+  const int lane = bg_lane();
+  const bool op = lane < NC;
+  uint32_t *own = tc.status + (size_t)tc.t * tc.n_carry;
+  const uint32_t *prev = own - tc.n_carry;
+  uint32_t w = 0, o = 0;
+  if (op) {
+    o = bg_st_load(own + lane);
+    w = bg_st_load(prev + lane);
+  }
+  const uint32_t okind = (o >> 2) & 3u;
+  const bool opub = op && (o >> 4) == ep && (okind == BG_ST_AGG || okind == BG_ST_INC);
+  const uint64_t unpub = ~__builtin_amdgcn_ballot_w64(opub);
+  const uint32_t n = unpub ? (uint32_t)__builtin_ctzll(unpub) : 64u;  // <= NC
+  const bool in = (uint32_t)lane < n;
+  const bool inc = in && (w >> 4) == ep && ((w >> 2) & 3u) == BG_ST_INC;
+  uint64_t cin = __builtin_amdgcn_ballot_w64(inc && (w & 1u));
+  uint64_t rest = ((n >= 64) ? ~0ull : ((1ull << n) - 1ull)) & ~__builtin_amdgcn_ballot_w64(inc);
+  while (rest) {
+    const int k = __builtin_ctzll(rest);
+    rest &= rest - 1;
+    const uint32_t wk = (uint32_t)__builtin_amdgcn_readlane((int)w, k);
+    cin |= (uint64_t)(bg_lookback(tc, (uint32_t)k, wk) & 1u) << k;
+  }
+  if (in && okind == BG_ST_AGG) bg_st_store(own + lane, bg_st_pack(tc.epoch, BG_ST_INC, (uint32_t)(cin >> lane) & 1u));
+  if (n >= (uint32_t)NC && cin == 0) return true;
+  tc.mode = BG_REPLAY;
+  tc.k_exact = n;
+  tc.cin = cin;
+  return false;
+}
+
 // MatchStar(M, C) = (((M & C) + C) ^ C) | M
 template <int L, int S, int K>
 __device__ __forceinline__ uint64_t bg_matchstar(uint64_t m, uint64_t c, bg_tile &tc, int bnd, int &infl) {
@@ -642,8 +720,9 @@ __device__ __forceinline__ bool bg_queue_take(const bg_queue &Q, uint32_t &t) {
 // atomic returns before the last block resets, and device-scope atomics are
 // performed at the device's coherence point in that order.  The control
 // words carry no other data, and the next launch on the stream starts after
-// this one's end-of-kernel release.
-__device__ __forceinline__ void bg_queue_close(const bg_queue &Q) {
+// this one's end-of-kernel release.  advance = false keeps the epoch for a
+// following launch over the same status words (the settle after a sweep).
+__device__ __forceinline__ void bg_queue_close(const bg_queue &Q, bool advance = true) {
   if (bg_lane() != 0) return;
   const uint32_t wpb = blockDim.x / BG_WAVE;
   const uint32_t e = __hip_atomic_fetch_add(&Q.lds->exits, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -656,5 +735,6 @@ __device__ __forceinline__ void bg_queue_close(const bg_queue &Q) {
   const uint32_t qd = __hip_atomic_fetch_add(Q.ctl + BG_CTL_QDONE, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (qd + 1 != Q.nq) return;
   __hip_atomic_store(Q.ctl + BG_CTL_QDONE, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(Q.ctl + BG_CTL_EPOCH, (Q.epoch + 1u) & 0x0FFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (advance)
+    __hip_atomic_store(Q.ctl + BG_CTL_EPOCH, (Q.epoch + 1u) & 0x0FFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -160,11 +160,14 @@ int bg_kernel_occupancy(bg_kernel_t kernel, uint32_t block_threads, int32_t *blo
 /* Words of a control block (BG_CTL_WORDS), for callers without this header. */
 uint32_t bg_ctl_words(void);
 
-/* Launch one generated group kernel over one buffer: persistent blocks of
- * 256 threads that take tiles from the control block's queues until none is
- * left (grid_blocks = 0: as many blocks as fit on the device at once, never
- * more than one wave per tile).  Replaces CUDA_Kernel.run
- * (cuda_kernel.py:411-427), asynchronously: the caller synchronises. */
+/* Launch one generated group kernel over one buffer: persistent blocks (of
+ * the size the kernel's launch bounds name) that take tiles from the control
+ * block's queues until none is left (grid_blocks = 0: as many blocks as fit
+ * on the device at once, never more than one wave per tile).  A kernel
+ * generated in the sweep + settle form (it comes with an empty marker
+ * kernel <name>_two_launch) is launched twice, with phase argument 0 and 1;
+ * others once.  Replaces CUDA_Kernel.run (cuda_kernel.py:411-427),
+ * asynchronously: the caller synchronises. */
 int bg_group_launch(bg_kernel_t kernel, const bg_group_args_t *args, uint32_t grid_blocks,
                     void *stream);
 
@@ -180,7 +183,8 @@ int bg_popcount(const uint64_t *src, uint64_t n_words, uint64_t *count, void *st
 /* Batched launch (no counterpart in the reference, which launches one grid
  * of (groups x inputs) blocks per pattern set): grid_blocks blocks (0 = as
  * many as fit at once) of block_threads threads of the persistent batch
- * kernel.  A single kernel node, so it can be captured in a graph and
+ * kernel (two, sweep then settle, for a kernel in that form, as in
+ * bg_group_launch).  Kernel nodes only, so it can be captured in a graph and
  * replayed (the control block resets itself at the end of every launch). */
 int bg_batch_launch(bg_kernel_t kernel, const bg_batch_args_t *args, uint32_t grid_blocks,
                     uint32_t block_threads, void *stream);

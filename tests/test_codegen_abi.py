@@ -96,6 +96,36 @@ def test_pass_loop_replays_exactly(regexes):
     assert f"bg_settle<{p.n_carry}>(tc)" in src
 
 
+def _gprog(regexes, **kw):
+    _, _, post, vnm = compiler.group_programs(regexes)
+    return GroupProgram(0, post, vnm, regexes, **kw)
+
+
+def test_two_launch_form_has_one_pass_per_launch():
+    """carry_settle="launch": no pass loop; the settle launch starts each tile
+    with bg_settle_begin (before any load), the sweep ends with bg_sweep_end;
+    the queue keeps its epoch after the sweep; the marker kernel tells the
+    runtime to launch twice, in the plain and in the batched form."""
+    from bitgen_amd.codegen.gen_hip import batch_source
+    p = _gprog(["ab[^\\n]*cd", "x.*y", "q[0-9]+z"], carry_settle="launch")
+    src = p.source()
+    assert p.two_launch and p.loop_start is None and p.meta()["launches"] == 2
+    assert "for (;;)" not in src and "bg_settle<" not in src
+    begin = src.index(f"if (phase != 0 && bg_settle_begin<{p.n_carry}>(tc)) return;")
+    assert begin < src.index("bas[i] = basis[")
+    assert f"if (!bg_sweep_end<{p.n_carry}>(tc)) return;" in src
+    assert src.count("goto bg_pass_end;") == p.n_carry
+    assert "bg_queue_close(Q, phase != 0);" in src
+    assert 'extern "C" __global__ void bg_group_0_two_launch() {}' in src
+    b = batch_source(p)
+    assert "bg_batch_0_two_launch()" in b and "bg_queue_close(Q, phase != 0);" in b
+    # programs that wait for every carry stay one launch
+    w = _gprog(["x.*y"], carry_settle="launch", lookback="wait")
+    assert not w.two_launch and "bg_queue_close(Q);" in w.source() and "_two_launch" not in w.source()
+    with pytest.raises(ValueError):
+        _gprog(["x.*y"], carry_settle="later")
+
+
 def test_carry_bounds_follow_data_flow():
     """Speculation bounds: a carry op's operand bound only involves the carry
     ops its operands are computed from."""

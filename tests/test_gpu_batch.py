@@ -102,3 +102,27 @@ def test_batched_bigram_filter(gpu):
     for s in sets:
         s.free()
 
+
+
+def test_batched_two_launch_graph(gpu):
+    """Batch kernels in the sweep + settle form: both launches of every
+    kernel are captured into the graph; eager and every replay bit-exact."""
+    import torch
+
+    sets = [BatchPatternSet(s, {"carry_settle": "launch", "groups_per_kernel": k}).load(0)
+            for s, k in zip(SETS, (1, 2))]
+    buffers = _buffers(7, 12)
+    jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers))]
+    print(f"buffers={[b.size for b in buffers]}", flush=True)
+    plan = BatchPlan(sets, buffers, jobs)
+    plan.enqueue()
+    _check(plan, sets, buffers, jobs)
+    plan.capture()
+    for rep in range(3):
+        plan.result.fill_(0x5A)
+        plan.run_graph()
+        torch.cuda.synchronize()
+        _check(plan, sets, buffers, jobs)
+    plan.destroy()
+    for s in sets:
+        s.free()

@@ -119,7 +119,8 @@ def test_plain_small_workload(gpu):
     assert O.popcount(ref) > 0
 
 
-def test_carry_crosses_many_tiles(gpu):
+@pytest.mark.parametrize("settle", ["inline", "launch"])
+def test_carry_crosses_many_tiles(gpu, settle):
     # one 300 KB line: MatchStar/ScanThru carries and the line-end projection
     # propagate through ~75 tiles
     body = np.full(300_000, ord("x"), dtype=np.uint8)
@@ -128,16 +129,23 @@ def test_carry_crosses_many_tiles(gpu):
     body[-1] = ord("\n")
     body[150_000] = ord("a")
     groups = [["a.*b", "a[^\\n]*b"], ["ax*b"], ["q"]]
-    ref = assert_parity(groups, body, gpu, label="long line")
+    ref = assert_parity(groups, body, gpu, options={"carry_settle": settle}, repeat=2, label=f"long line {settle}")
     assert O.popcount(ref[0]) == 1 and O.popcount(ref[1]) == 1 and O.popcount(ref[2]) == 0
 
 
-@pytest.mark.parametrize("lookback", ["speculate", "wait"])
+CARRY_FORMS = {"speculate": {"lookback": "speculate", "carry_settle": "inline"},
+               "wait": {"lookback": "wait"},
+               "launch": {"lookback": "speculate", "carry_settle": "launch"}}
+
+
+@pytest.mark.parametrize("lookback", list(CARRY_FORMS))
 def test_carry_runs_through_thousands_of_tiles(gpu, lookback):
     """A 24 MiB line (no LF until the end): the line-end projection and the
     stars propagate through ~6,000 tiles, so every tile's carry-in is decided
     by a predecessor thousands of tiles back (the look-back reads 64 statuses
-    per step).  Matches start near the front, the middle and the end."""
+    per step).  Matches start near the front, the middle and the end.  In
+    the two-launch form every tile publishes an aggregate in the sweep and
+    the settle resolves the whole chain."""
     n = 24 * 2**20
     body = np.full(n, ord("x"), dtype=np.uint8)
     body[[5, n // 2, n - 300]] = ord("a")
@@ -145,7 +153,7 @@ def test_carry_runs_through_thousands_of_tiles(gpu, lookback):
     body[-1] = ord("\n")
     groups = [["a.*b", "a[^\n]*b"], ["ax*b"], ["q"], ["x{3}b"]]
     print(f"n={n} lookback={lookback}", flush=True)
-    ref = assert_parity(groups, body, gpu, options={"lookback": lookback}, label=f"24 MiB line {lookback}")
+    ref = assert_parity(groups, body, gpu, options=CARRY_FORMS[lookback], repeat=2, label=f"24 MiB line {lookback}")
     assert [O.popcount(r) for r in ref] == [1, 1, 0, 1]
 
 
@@ -199,3 +207,29 @@ def test_more_than_64_carry_ops(gpu):
     alpha = np.frombuffer(b"abcdefghij0123456789 \n", dtype=np.uint8)
     data = rng.choice(alpha, size=150_000)
     assert_parity([regs], data, gpu, label="70 stars")
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_two_launch_settle(gpu, seed):
+    """carry_settle="launch": a sweep launch that never waits at the end of a
+    tile, then a settle launch that replays only tiles whose assumed
+    carry-ins were wrong or whose sweep aborted (deferred statuses).  Star-
+    and loop-heavy groups, short and long lines, single, merged and filtered
+    kernels, stream-level groups; each run twice (the second run's epoch
+    must not see the first run's statuses)."""
+    rng = random.Random(400 + seed)
+    alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
+    p = [0.96 / 12] * 12 + [0.04] if seed % 2 else [1 / 13] * 13
+    n = rng.choice([70, 9000, 70000, 300001, 2_000_003])
+    data = np.random.default_rng(400 + seed).choice(alpha, size=n, p=p)
+    for _ in range(rng.randint(0, 6)):  # long runs that carries cross
+        run = rng.choice([b"bc", b"ab", b"x", b"a"]) * rng.randint(100, 20000)
+        off = rng.randrange(0, n)
+        seg = np.frombuffer(run[:n - off], dtype=np.uint8)
+        data[off:off + seg.size] = seg
+    pool = POOL if seed % 3 == 0 else FUSED_POOL
+    groups = [rng.sample(pool, rng.randint(1, 4)) for _ in range(rng.randint(1, 6))]
+    opts = {"carry_settle": "launch", "groups_per_kernel": 1 + seed % 4 if pool is FUSED_POOL else 1,
+            "bigram_filter": seed % 2 if pool is FUSED_POOL else 0, "min_waves": 5 if seed % 4 == 3 else 0}
+    print(f"seed={seed} n={n} opts={opts} groups={groups}", flush=True)
+    assert_parity(groups, data, gpu, options=opts, repeat=2, label=f"two-launch seed {seed}")
