@@ -28,6 +28,7 @@ import concurrent.futures
 import json
 import math
 import os
+import pprint
 
 from .gen import Generator
 from ..inst import BsInst, BsInstType
@@ -1132,7 +1133,8 @@ class HipGenerator(Generator):
             f.write('"""Generated by bitgen_amd.codegen.gen_hip (hip backend module)."""\n')
             f.write("import os\n")
             f.write("from bitgen_amd.backend.hip.template import init_kernel as _init_kernel\n\n")
-            f.write(f"GROUPS = {json.dumps(metas, indent=1)}\n")
+            # a Python literal (metas hold None for stream-level groups)
+            f.write(f"GROUPS = {pprint.pformat(metas, width=110)}\n")
             f.write("HERE = os.path.dirname(os.path.abspath(__file__))\n")
             f.write(f"grid_size = ({len(metas)}, {cfg.get_config('multi_input', 1)}, 1)\n")
             f.write(f"block_size = {tuple(cfg.get_config('block_size', (256, 1, 1)))}\n\n\n")

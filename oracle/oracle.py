@@ -256,9 +256,9 @@ def mask_to_positions(words, n_positions):
     """Clear every bit at or beyond n_positions."""
     words = np.array(words, dtype=np.uint64, copy=True)
     W = words.shape[-1]
-    full = n_positions // 64
+    full = int(n_positions) // 64  # Python ints: 1 << 63 must not wrap
     if full < W:
-        rem = n_positions % 64
+        rem = int(n_positions) % 64
         words[..., full] &= np.uint64((1 << rem) - 1) if rem else np.uint64(0)
         words[..., full + 1:] = 0
     return words

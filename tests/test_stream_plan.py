@@ -122,3 +122,34 @@ def test_generator_plugin_routes_loops_to_stream_level():
     m = progs[0].meta()
     assert m["kind"] == "stream" and m["plan"][1][0] == "loop"
     assert all(f"bg_group_0_s{i}" in progs[0].source() for i in range(len(m["step_kernels"])))
+
+
+def test_generated_module_of_a_stream_group_is_valid_python(tmp_path):
+    """lower() writes kernels/kernel_0.py with the kernel metas as a Python
+    literal; a stream-level group's meta holds None fields (no single
+    kernel), which must survive as None (compiled for gfx950 here, nothing
+    launched)."""
+    import ast
+    import os
+
+    from bitgen_amd import config as cfg
+    from bitgen_amd.codegen import create_generator
+    from bitgen_amd.tool.xml_lexical import xml_lexical_pablo
+
+    xml, xvnm = compiler.parse_pablo(xml_lexical_pablo())
+    xml, xvnm = compiler.apply_passes(xml, xvnm, {})
+    saved = dict(cfg.get_config_dict())
+    cwd = os.getcwd()
+    try:
+        os.chdir(tmp_path)
+        cfg.set_config("__cached_module_path", "code")
+        cfg.set_config("use_cached_code", False)
+        path = create_generator("hip")([xml], [xvnm], regex_list=[["xml_parser"]]).lower()
+        src = open(path).read()
+    finally:
+        os.chdir(cwd)
+        cfg.set_from_dict(saved)
+    compile(src, path, "exec")
+    groups = ast.literal_eval(src.split("GROUPS = ", 1)[1].split("\nHERE = ", 1)[0])
+    assert groups[0]["kind"] == "stream" and groups[0]["kernel"] is None
+    assert groups[0]["hsaco"] == "kernel_0.hsaco"
