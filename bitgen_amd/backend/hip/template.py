@@ -14,7 +14,13 @@ tmp_streams, kernel_handle, run, dyn_stats)`` where
   current torch stream and synchronises it, like CUDA_Kernel.run
   (cuda_kernel.py:411-427).  Stores are plain (not ``|=``), so the caller
   need not zero ``result`` first;
-* ``dyn_stats``: None.
+* ``dyn_stats``: None, or (kernels compiled with ``inst_stats_dyn``, the
+  reference's ``--pass-inst-stats-dyn``) int64 ``[1, M, K, 1 + T]`` filled by
+  every ``run``: ``[..., 0]`` passes summed over the kernel's tiles (1 per
+  tile, more where carries forced replays; the reference's iteration
+  count), ``[..., 1 + t]`` zero / byte-pair tests that fell through in tile
+  t's exact pass (the dynamic work per tile; T = the most tiles of any
+  kernel, shorter rows are zero-padded).
 """
 import os
 
@@ -211,6 +217,24 @@ class HipGroupSet:
             execute(meta["plan"])
             result_row.copy_(ws[meta["result_slot"]])
 
+    def dyn_stats(self, status, n_words):
+        """Per-tile statistics of kernels compiled with ``inst_stats_dyn``
+        (see the module docstring): int64 [M, K, 1 + T] on the device, from
+        the statistics slot (index n_carry) of each tile's status words."""
+        torch = _torch()
+        M = status.size(0)
+        tiles = [self.n_tiles(m, n_words) if m.get("dyn_stats") else 0 for m in self.metas]
+        out = torch.zeros((M, self.n_kernels, 1 + max(tiles + [0])), dtype=torch.int64, device=status.device)
+        for k, m in enumerate(self.metas):
+            T = tiles[k]
+            if not T:
+                continue
+            C = m["status_slots"]
+            words = status[:, k, :T * C].reshape(M, T, C)[:, :, C - 1].to(torch.int64) & 0xFFFFFFFF
+            out[:, k, 1:1 + T] = words & 0xFFFFF
+            out[:, k, 0] = (words >> 20).sum(dim=1)
+        return out
+
     def copy_errors(self, stream=None):
         """Enqueue a copy of the kernels' error flags into pinned host memory
         (after the launches on the same stream); ``raise_errors`` reads it once
@@ -264,11 +288,16 @@ def init_kernel(groups_meta, base_dir, regex_func_num, input_stream_tensor, basi
     M, W = basis.size(1), basis.size(3)
     result = torch.zeros((1, M, ks.n_groups, W), dtype=torch.uint64, device=basis.device)
     status = ks.alloc_status(M, W)
+    dyn = None
+    if any(m.get("dyn_stats") for m in groups_meta):
+        dyn = ks.dyn_stats(status, W).unsqueeze(0)
 
     def run(basic_stream_tensor):
         ks.run(basic_stream_tensor, result, status)
+        if dyn is not None:
+            dyn[0].copy_(ks.dyn_stats(status, W))
         ks.copy_errors()
         rt.stream_sync(rt.current_stream_ptr())
         ks.raise_errors()
 
-    return result, status, ks, run, None
+    return result, status, ks, run, dyn

@@ -128,7 +128,8 @@ class GroupProgram:
     """Analysis + HIP source of one group program."""
 
     def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
-                 min_waves=0, bigram_filter=False, replay_scope="stage", block_threads=256, carry_settle="inline"):
+                 min_waves=0, bigram_filter=False, replay_scope="stage", block_threads=256, carry_settle="inline",
+                 inst_stats_dyn=False):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -153,7 +154,10 @@ class GroupProgram:
         at the end of its pass, waiting for its predecessor, and replays in
         a loop; "launch" -- the kernel runs twice, a sweep that never waits
         and a settle that replays only the tiles that need it (one pass per
-        tile and launch, so no pass loop)."""
+        tile and launch, so no pass loop).  inst_stats_dyn: each tile also
+        writes one statistics word into an extra status slot (index
+        n_carry): passes run (bits 20..31) and zero / byte-pair tests that
+        fell through in its exact pass (bits 0..19)."""
         if block_threads % WAVE or not 64 <= block_threads <= 1024:
             raise ValueError(f"block_threads must be a multiple of 64 in 64..1024, not {block_threads}")
         self.block_threads = int(block_threads)
@@ -165,6 +169,7 @@ class GroupProgram:
         if carry_settle not in ("inline", "launch"):
             raise ValueError(f"carry_settle must be 'inline' or 'launch', not {carry_settle!r}")
         self.carry_settle = carry_settle
+        self.inst_stats_dyn = bool(inst_stats_dyn)
         self.gid = gid
         self.insts = insts
         self.vnm = vnm
@@ -611,6 +616,10 @@ class GroupProgram:
                 feeds.setdefault(j, []).append((k, math.ceil(bits / WAVE)))
         if speculative:
             add("  int bg_infl = -1; uint32_t bg_vseen = 0;" + "".join(f" int bg_bnd_{k} = -1;" for k in sorted(consumers)))
+        stats = self.inst_stats_dyn
+        if stats:
+            # passes in bits 20..31, tests passed in this pass in bits 0..19
+            add("  uint32_t bg_dyn = " + ("0u;" if looped else "1u << 20;"))
         # three-input bitwise trees -> v_bitop3_b32
         trees, absorbed = self._fuse()
         self.fused = (trees, absorbed)
@@ -638,6 +647,8 @@ class GroupProgram:
                     add("  for (;;) {")
                 if self.bigram_keys:
                     add("  " + " ".join(f"bg_sfence(bgm[{c}]);" for c in range(2 * ((len(self.bigram_keys) + 63) // 64))))
+                if stats:
+                    add("  bg_dyn = (bg_dyn & 0xFFF00000u) + (1u << 20);")
                 if not tile_scope:
                     zeros = [f"{_cname(v)} = 0;" for v in body_vars]
                     for i in range(0, len(zeros), 16):
@@ -679,12 +690,16 @@ class GroupProgram:
                 if self.expect_zero:
                     test = f"__builtin_expect({test}, 1)"
                 add(f"  if ({test}) goto L{inst.label};")
+                if stats:
+                    add("  bg_dyn++;")
                 open_regions.add(inst.label)
             elif t == BsInstType.SKIPZERO:
                 test = f"bg_wave_zero({self.val(inst.operands[0])})"
                 if self.expect_zero:
                     test = f"__builtin_expect({test}, 1)"
                 add(f"  if ({test}) goto L{inst.label};")
+                if stats:
+                    add("  bg_dyn++;")
                 open_regions.add(inst.label)
             elif t == BsInstType.LABEL:
                 add(f"L{inst.label}:;")
@@ -724,7 +739,19 @@ class GroupProgram:
             add("  }")
         elif two:
             add("bg_pass_end:;")
-            add(f"  if (!bg_sweep_end<{self.n_carry}>(tc)) return;")
+            add(f"  const bool bg_keep = bg_sweep_end<{self.n_carry}>(tc);")
+        if stats:
+            # one word per tile after its carry statuses; a settle replay
+            # keeps the sweep's pass count and replaces the test count
+            add("  if (lane == 0) {")
+            add(f"    uint32_t *bg_dw = status + (size_t)t * n_carry + {self.n_carry};")
+            if two:
+                add("    *bg_dw = phase != 0 ? (*bg_dw & 0xFFF00000u) + bg_dyn : (bg_keep ? bg_dyn : bg_dyn & 0xFFF00000u);")
+            else:
+                add("    *bg_dw = bg_dyn;")
+            add("  }")
+        if two:
+            add("  if (!bg_keep) return;")
         add("  const int lo = (t == 0) ? 0 : HL;")
         add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
         add("  if (inb && lane >= lo && lane < hi) {")
@@ -968,33 +995,84 @@ class GroupProgram:
 
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
+        call = (f"bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
+                "a.ctl + BG_CTL_ERROR, bgt, phase);")
+        chunk = [
+            "const uint32_t tl = g + (uint32_t)bg_lane();",
+            "uint32_t *row = tl < a.n_tiles ? a.status + (size_t)tl * a.n_carry : nullptr;",
+            "bg_tile tc0 = bg_tile_make(g, 0, (int64_t)a.n_words, a.status, a.n_carry, Q.epoch, a.ctl + BG_CTL_ERROR);",
+            "const uint32_t *prev = a.status + (size_t)(g ? g - 1 : 0) * a.n_carry;",
+            f"need = bg_settle_chunk<{self.n_carry}>(row, tl == 0, prev, tc0);",
+        ]
         return "\n".join([
             '#include "bitgen_device.h"',
             self.device_function(),
             f'extern "C" __global__ {self.launch_bounds(self.block_threads)} void {self.kernel_name}(bg_group_args_t a, '
             'uint32_t phase) {',
-            "  // persistent: each wave takes tiles from its block's queue until the",
-            "  // queue is exhausted",
-            "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.n_tiles, a.take);",
             *self.table_decl(self.block_threads),
-            "  uint32_t first;",
-            "  bool more = bg_queue_take(Q, first);",
-            "  while (more) {",
-            "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
-            "    for (uint32_t i = 0, t = first; i < Q.take && t < Q.n; i++, t += Q.nq)",
-            f"      bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
-            "a.ctl + BG_CTL_ERROR, bgt, phase);",
-            "    more = bg_queue_finish(Q, next, first);",
-            "  }",
-            self.close_line(),
+            *self.work_loop("a.n_tiles", [call], chunk),
             "}",
             *self.two_launch_marker(self.kernel_name),
             "",
         ])
 
-    def close_line(self):
-        # a sweep keeps the epoch: its settle launch reads the same statuses
-        return "  bg_queue_close(Q, phase != 0);" if self.two_launch else "  bg_queue_close(Q);"
+    def work_loop(self, n_tiles, run, chunk):
+        """Kernel body: persistent waves take tiles from the control block's
+        queues (the next take issued before the current tile runs) and run
+        each (`run`: statements for tile t, phase `phase`; one call site, as
+        the tile program is inlined).  Two-launch form, settle launch
+        (phase 1): waves take chunks of 64 consecutive tiles from 8 queues,
+        resolve a chunk's carry-ins at once (`chunk`: statements setting
+        `need` for tiles g .. g + 63) and run only the tiles in `need`, in
+        order."""
+        ind = lambda xs, n: [" " * n + x for x in xs]
+        if not self.two_launch:
+            return [
+                "  // persistent: each wave takes tiles from its block's queue until the",
+                "  // queue is exhausted",
+                f"  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, {n_tiles}, a.take);",
+                "  uint32_t first;",
+                "  bool more = bg_queue_take(Q, first);",
+                "  while (more) {",
+                "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
+                "    for (uint32_t i = 0, t = first; i < Q.take && t < Q.n; i++, t += Q.nq) {",
+                *ind(run, 6),
+                "    }",
+                "    more = bg_queue_finish(Q, next, first);",
+                "  }",
+                "  bg_queue_close(Q);",
+            ]
+        return [
+            "  // sweep (phase 0): one tile per take from the launch's queues;",
+            "  // settle (phase 1): chunks of 64 tiles from 8 queues",
+            f"  const bg_queue Q = phase == 0 ? bg_queue_open(a.ctl, a.n_queues, {n_tiles}, 1)",
+            f"                                : bg_queue_open(a.ctl, 8, ({n_tiles} + 63u) / 64u, 1);",
+            "  uint32_t cur;",
+            "  bool more = bg_queue_take(Q, cur);",
+            "  uint64_t need = 0;",
+            "  uint32_t g = 0, next = 0;",
+            "  while (true) {",
+            "    uint32_t t;",
+            "    if (phase == 0) {",
+            "      if (!more) break;",
+            "      t = cur;",
+            "      next = bg_queue_issue(Q);  // next tile, taken ahead",
+            "    } else {",
+            "      while (need == 0 && more) {",
+            "        g = cur * 64u;",
+            "        next = bg_queue_issue(Q);",
+            *ind(chunk, 8),
+            "        more = bg_queue_finish(Q, next, cur);",
+            "      }",
+            "      if (need == 0) break;",
+            "      t = g + (uint32_t)__builtin_ctzll(need);",
+            "      need &= need - 1;",
+            "    }",
+            *ind(run, 4),
+            "    if (phase == 0) more = bg_queue_finish(Q, next, cur);",
+            "  }",
+            "  bg_queue_close(Q, phase != 0);",
+        ]
 
     def two_launch_marker(self, kernel):
         """An empty kernel named <kernel>_two_launch: its presence in the code
@@ -1012,7 +1090,9 @@ class GroupProgram:
             "S": self.S,
             "n_carry": self.n_carry,
             # look-back status words per tile: one per carry op (>= 1)
-            "status_slots": max(1, self.n_carry),
+            "status_slots": self.n_carry + 1 if self.inst_stats_dyn else max(1, self.n_carry),
+            # statistics word of each tile at status slot n_carry
+            "dyn_stats": self.inst_stats_dyn,
             # launches per run: 2 for the sweep + settle form
             "launches": 2 if self.two_launch else 1,
             "n_out": self.n_out,
@@ -1032,26 +1112,36 @@ def batch_source(program, block_threads=256):
     through ``tile_job``; each job's tiles are contiguous and increasing, so a
     tile's predecessor always has a smaller index)."""
     g = program.gid
+    run = [
+        "const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[t]);",
+        "const bg_job_t J = a.jobs[j];",
+        f"bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, t - J.tile_start, Q.epoch, J.n_carry, "
+        "a.ctl + BG_CTL_ERROR, bgt, phase);",
+    ]
+    chunk = [
+        "// lane i: global tile g + i, its job and row; lane 0's job for the",
+        "// look-back before the chunk",
+        "const uint32_t tl = g + (uint32_t)bg_lane();",
+        "uint32_t *row = nullptr;",
+        "bool first = false;",
+        "if (tl < a.total_tiles) {",
+        "  const bg_job_t &Jl = a.jobs[a.tile_job[tl]];",
+        "  row = Jl.status + (size_t)(tl - Jl.tile_start) * Jl.n_carry;",
+        "  first = tl == Jl.tile_start;",
+        "}",
+        "const bg_job_t J0 = a.jobs[(uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[g])];",
+        "const uint32_t t0 = g - J0.tile_start;",
+        "bg_tile tc0 = bg_tile_make(t0, 0, (int64_t)J0.n_words, J0.status, J0.n_carry, Q.epoch, a.ctl + BG_CTL_ERROR);",
+        "const uint32_t *prev = J0.status + (size_t)(t0 ? t0 - 1 : 0) * J0.n_carry;",
+        f"need = bg_settle_chunk<{program.n_carry}>(row, first, prev, tc0);",
+    ] if program.two_launch else []
     return "\n".join([
         '#include "bitgen_device.h"',
         program.device_function(),
         f'extern "C" __global__ {program.launch_bounds(block_threads)} void {batch_kernel_name(g)}(bg_batch_args_t a, '
         'uint32_t phase) {',
-        "  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, a.total_tiles, a.take);",
         *program.table_decl(block_threads),
-        "  uint32_t first;",
-        "  bool more = bg_queue_take(Q, first);",
-        "  while (more) {",
-        "    const uint32_t next = bg_queue_issue(Q);  // next tiles, taken ahead",
-        "    for (uint32_t i = 0, tg = first; i < Q.take && tg < Q.n; i++, tg += Q.nq) {",
-        "      const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[tg]);",
-        "      const bg_job_t J = a.jobs[j];",
-        f"      bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, tg - J.tile_start, Q.epoch, J.n_carry, "
-        "a.ctl + BG_CTL_ERROR, bgt, phase);",
-        "    }",
-        "    more = bg_queue_finish(Q, next, first);",
-        "  }",
-        program.close_line(),
+        *program.work_loop("a.total_tiles", run, chunk),
         "}",
         *program.two_launch_marker(batch_kernel_name(g)),
         "",
@@ -1093,17 +1183,23 @@ class HipGenerator(Generator):
         """One program per group: a fused GroupProgram, or a StreamGroup when
         the group needs whole streams between its steps (graph breaks,
         while loops)."""
+        from .. import compiler
         from .gen_stream import StreamGroup, needs_stream_plan
 
+        # the library's launch options; the reference's --pass-inst-stats-dyn
+        # asks for per-tile statistics (gen_cuda.py:152-172)
+        opts = dict(compiler.DEFAULT_OPTIONS, inst_stats_dyn=int(bool(cfg.get_config("pass_inst_stats_dyn"))))
+        step_opts = dict(opts, inst_stats_dyn=0, bigram_filter=0)
         self.programs = []
         for g, (insts, vnm) in enumerate(zip(self.insts_list, self.var_name_map_list)):
             regs = self.regex_list[g] if self.regex_list else None
             if isinstance(regs, str):
                 regs = [regs]
             if needs_stream_plan(insts):
-                self.programs.append(StreamGroup(g, insts, vnm, regs, lambda gid, i, v, r=regs: GroupProgram(gid, i, v, r)))
+                self.programs.append(StreamGroup(
+                    g, insts, vnm, regs, lambda gid, i, v, r=regs: compiler.program_for(gid, i, v, r or [], step_opts)))
             else:
-                self.programs.append(GroupProgram(g, insts, vnm, regs))
+                self.programs.append(compiler.program_for(g, insts, vnm, regs or [], opts))
         return self.programs
 
     def lower(self, input_path=None):

@@ -53,7 +53,10 @@ DEFAULT_OPTIONS = {
     # pass, waiting for its predecessor) or by a second "launch" of the
     # kernel (a sweep that never waits, then a settle that replays only the
     # tiles that need it)
-    "carry_settle": "inline",
+    "carry_settle": "launch",
+    # per-tile statistics word (passes, tests passed) after the carry
+    # statuses (the reference's --pass-inst-stats-dyn)
+    "inst_stats_dyn": 0,
 }
 
 
@@ -210,7 +213,8 @@ def program_for(gid, insts, vnm, regexes, options):
                         expect_zero=bool(options["sched_expect"]), pipeline=bool(options["sched_pipeline"]),
                         min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]),
                         replay_scope=options["replay_scope"], block_threads=int(options.get("block_threads", 256)),
-                        carry_settle=options.get("carry_settle", "inline"))
+                        carry_settle=options.get("carry_settle", "inline"),
+                        inst_stats_dyn=bool(options.get("inst_stats_dyn", 0)))
 
 
 @dataclass
@@ -320,7 +324,7 @@ def kernel_programs(groups, o):
         programs = []
         for g, (insts, vnm) in enumerate(posts):
             if needs_stream_plan(insts):
-                step_opts = dict(o, bigram_filter=0)
+                step_opts = dict(o, bigram_filter=0, inst_stats_dyn=0)
 
                 def make(gid, step_insts, step_vnm, regs=groups[g], so=step_opts):
                     return program_for(gid, step_insts, step_vnm, regs, so)

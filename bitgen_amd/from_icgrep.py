@@ -245,6 +245,19 @@ def run_regex_kernel(kernel_path, regex, input_path, input_stream_tensor, basic_
         ok = validation(regex if isinstance(regex, list) else regex, data, bs,
                         as_one_group=cfg.get_config("validation_as_one_group", False))
         global_timer.append_data("run_regex", "check", ok)
+    if cfg.get_config("pass_inst_stats_dyn") and dyn_stats is not None:
+        # the reference's report of its dynamic statistics (from_icgrep.py:619-640):
+        # here iterations = passes over tiles, offsets = tests passed per tile
+        iter_num = dyn_stats[..., :1]
+        dyn_offset_num = dyn_stats[..., 1:]
+        print("iter_num:", tuple(iter_num.shape), iter_num.sum().item())
+        print("dyn_offset_num:", tuple(dyn_offset_num.shape), dyn_offset_num.sum().item())
+        res = global_timer.inst_result
+        res["iter_num_total"] = res.get("iter_num_total", 0) + iter_num.sum().item()
+        res["dyn_offset_num_max"] = res.get("dyn_offset_num_max", 0) + dyn_offset_num.to(torch.float32).max().item()
+        res["dyn_offset_num_std"] = dyn_offset_num.to(torch.float32).std().item()
+        res["dyn_offset_num_total"] = res.get("dyn_offset_num_total", 0) + dyn_stats.sum().item()
+        print(res)
     torch.cuda.synchronize()
     if kernel_handle is not None:
         kernel_handle.free()

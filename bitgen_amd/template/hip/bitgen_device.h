@@ -613,6 +613,71 @@ __device__ __forceinline__ bool bg_settle_begin(bg_tile &tc) {
   return false;
 }
 
+// Settle launch, one chunk of 64 consecutive tiles at once (lane i = tile
+// g + i; a chunk may span buffers in the batched form).  row: this lane's
+// status row (nullptr: no tile); first: the tile is its buffer's tile 0
+// (carry-in 0); prev: the status row of lane 0's predecessor (unused when
+// lane 0's tile is first); tc0: lane 0's tile (for the look-back behind an
+// aggregate before the chunk).
+//
+// The carry into a tile is the carry-out of the tile below it, and an
+// aggregate's carry-out is its own carry-in; so the carry into lane i is the
+// value of the nearest lane j <= i whose predecessor status is inclusive
+// (or its buffer's start: 0), or the chunk's incoming carry when every
+// predecessor down to lane 0 is an aggregate.  Per op: one load per lane
+// (all NC issued together), three ballots and a bit scan per lane; the
+// aggregates of the chunk publish their inclusive value.  Returns the tiles
+// that still need the per-tile settle (bg_settle_begin): a carry-in of 1,
+// an aborted sweep, or a carry-in decided by an aborted predecessor.
+template <int NC>
+__device__ __forceinline__ uint64_t bg_settle_chunk(uint32_t *row, bool first, const uint32_t *prev, bg_tile &tc0) {
+  static_assert(NC >= 1 && NC <= 64, "the two-launch form needs 1..64 carry operations");
+  const uint32_t ep = tc0.epoch & 0x0FFFFFFFu;
+  const uint32_t inc0 = bg_st_pack(tc0.epoch, BG_ST_INC, 0u);
+  const int lane = bg_lane();
+  const bool valid = row != nullptr;
+  const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const bool first0 = __builtin_amdgcn_readfirstlane((int)first) != 0;
+  uint32_t pw = inc0;  // lane k: lane 0's predecessor status of op k
+  if (!first0 && lane < NC) pw = bg_st_load(prev + lane);
+  uint32_t o[NC];
+#pragma unroll
+  for (int k = 0; k < NC; k++) o[k] = valid ? bg_st_load(row + k) : inc0;
+  bool need = false;
+#pragma unroll
+  for (int k = 0; k < NC; k++) {
+    // predecessor's status of op k: lane i - 1's own word (lane 0: pw)
+    uint32_t p = (uint32_t)__builtin_amdgcn_mov_dpp((int)o[k], 0x138 /*wave_shr:1*/, 0xF, 0xF, true);
+    const uint32_t p0 = (uint32_t)__builtin_amdgcn_readlane((int)pw, k);
+    if (lane == 0) p = p0;
+    if (first) p = inc0;
+    const bool pok = (p >> 4) == ep;
+    const uint32_t pk = (p >> 2) & 3u;
+    const uint64_t NA = ~__builtin_amdgcn_ballot_w64(pok && pk == BG_ST_AGG);
+    const uint64_t ONE = __builtin_amdgcn_ballot_w64(pok && pk == BG_ST_INC && (p & 1u));
+    const uint64_t UNK = __builtin_amdgcn_ballot_w64(!(pok && (pk == BG_ST_AGG || pk == BG_ST_INC)));
+    // the chunk's incoming carry, needed only behind an aggregate in lane 0
+    uint32_t cin_chunk = 0;
+    if (!(NA & 1ull)) cin_chunk = bg_lookback(tc0, (uint32_t)k, p0) & 1u;
+    const uint64_t m = NA & upto;
+    bool cin, unk;
+    if (m) {
+      const int j = 63 - __builtin_clzll(m);
+      cin = (ONE >> j) & 1ull;
+      unk = (UNK >> j) & 1ull;
+    } else {
+      cin = cin_chunk;
+      unk = false;
+    }
+    const bool ook = (o[k] >> 4) == ep;
+    const uint32_t okind = (o[k] >> 2) & 3u;
+    const bool odef = !(ook && (okind == BG_ST_AGG || okind == BG_ST_INC));
+    if (valid && ook && okind == BG_ST_AGG && !unk) bg_st_store(row + k, bg_st_pack(tc0.epoch, BG_ST_INC, cin ? 1u : 0u));
+    need = need || (valid && (odef || unk || cin));
+  }
+  return __builtin_amdgcn_ballot_w64(need);
+}
+
 // MatchStar(M, C) = (((M & C) + C) ^ C) | M
 template <int L, int S, int K>
 __device__ __forceinline__ uint64_t bg_matchstar(uint64_t m, uint64_t c, bg_tile &tc, int bnd, int &infl) {

@@ -10,6 +10,7 @@ import pytest
 from bitgen_amd import compiler, workloads
 from bitgen_amd.codegen import GroupProgram, HipCodegenError
 from bitgen_amd.csrc import build as nb
+from bitgen_amd.inst import BsInstType
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -113,7 +114,8 @@ def test_two_launch_form_has_one_pass_per_launch():
     assert "for (;;)" not in src and "bg_settle<" not in src
     begin = src.index(f"if (phase != 0 && bg_settle_begin<{p.n_carry}>(tc)) return;")
     assert begin < src.index("bas[i] = basis[")
-    assert f"if (!bg_sweep_end<{p.n_carry}>(tc)) return;" in src
+    assert f"const bool bg_keep = bg_sweep_end<{p.n_carry}>(tc);" in src
+    assert src.index("if (!bg_keep) return;") < src.index("result[w] = out0;")
     assert src.count("goto bg_pass_end;") == p.n_carry
     assert "bg_queue_close(Q, phase != 0);" in src
     assert 'extern "C" __global__ void bg_group_0_two_launch() {}' in src
@@ -499,3 +501,48 @@ def test_bigram_table_slot_matches_hash():
             key = (a << 8) | b
             h = bigram_hash(a, b)
             assert (key >> 5) & 0x3FF == h >> 5 and key & 31 == h & 31
+
+
+def test_inst_stats_dyn_slot_and_counting():
+    """inst_stats_dyn: one more status slot per tile (index n_carry) holding
+    the statistics word; every zero / byte-pair test that falls through
+    counts one, each pass adds 1 << 20; the word is written by lane 0 before
+    the stores, in both settle forms; without the option nothing changes."""
+    regs = ["ab[^\\n]*cd", "x.*y", "hello", "q[0-9]+z"]
+    for settle in ("inline", "launch"):
+        p = _gprog(regs, carry_settle=settle, inst_stats_dyn=True, bigram_filter=True)
+        src = p.source()
+        m = p.meta()
+        assert m["status_slots"] == p.n_carry + 1 and m["dyn_stats"] is True
+        from bitgen_amd.codegen.gen_hip import BigramTest
+        n_tests = sum(1 for i in p.insts if isinstance(i, BigramTest) or i.type == BsInstType.SKIPZERO)
+        assert any(isinstance(i, BigramTest) for i in p.insts)
+        assert src.count("bg_dyn++;") == n_tests > 0
+        assert f"uint32_t *bg_dw = status + (size_t)t * n_carry + {p.n_carry};" in src
+        assert src.index("bg_dw") < src.index("result[w] = out0;")
+        if settle == "inline":
+            assert "bg_dyn = (bg_dyn & 0xFFF00000u) + (1u << 20);" in src  # each pass of the loop
+        else:
+            assert "uint32_t bg_dyn = 1u << 20;" in src and "if (!bg_keep) return;" in src
+    q = _gprog(regs)
+    assert "bg_dyn" not in q.source() and q.meta()["status_slots"] == max(1, q.n_carry)
+    assert q.meta()["dyn_stats"] is False
+
+
+def test_tile_model_counts_tests_that_fall_through():
+    """The tile model's per-tile test count (the GPU statistics' reference):
+    a tile of text without any of a literal's bytes passes no test of it."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from tests.tile_model import run_tiles
+
+    p = _gprog(["zqxwvu"])
+    n_tests = sum(1 for i in p.insts if i.type == BsInstType.SKIPZERO)
+    data = np.full(20000, ord("a"), dtype=np.uint8)
+    data[15000:15006] = np.frombuffer(b"zqxwvu", dtype=np.uint8)
+    stats = []
+    run_tiles(p, O.transpose(data, O.n_words_for(data.size)), stats)
+    assert len(stats) == p.n_tiles(O.n_words_for(data.size))
+    # every test of the literal passes in the tile holding it, none elsewhere
+    assert sorted(stats) == [0] * (len(stats) - 1) + [n_tests] and n_tests > 0

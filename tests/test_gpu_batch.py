@@ -63,7 +63,9 @@ def test_batched_matches_oracle_eager_and_graph(gpu):
 
 
 def test_batched_partial_job_list(gpu):
-    sets = [BatchPatternSet(s).load(0) for s in SETS]
+    """A subset of the (set, buffer) pairs, few persistent blocks (many tiles
+    per wave), carries settled inline (one launch per kernel)."""
+    sets = [BatchPatternSet(s, {"carry_settle": "inline"}).load(0) for s in SETS]
     buffers = _buffers(2, 9)
     jobs = [(0, 0), (0, 3), (1, 3), (1, 8), (0, 8)]
     plan = BatchPlan(sets, buffers, jobs, grid_blocks=7)  # few persistent blocks: many tiles per wave

@@ -98,3 +98,21 @@ def test_generated_module_contract_and_positions(gpu, fresh_config):
     assert np.array_equal(a, b)
     assert Bitstream(r32, n_positions=len(data) + 1).get_count() == KAT["count"]
     handle.free()
+
+
+def test_run_bitgen_inst_stats_dyn(gpu, fresh_config, capsys):
+    """--pass-inst-stats-dyn 1 (the reference's dynamic statistics,
+    from_icgrep.py:619-640): same known answer, the statistics report is
+    printed and the per-tile statistics cover every tile once at least."""
+    from bitgen_amd import run_bitgen
+    from bitgen_amd.tool.timer import global_timer
+
+    global_timer.inst_result = {}
+    ok = run_bitgen.main(quick_start_argv() + ["--pass-inst-stats-dyn", "1"])
+    out = capsys.readouterr().out
+    print(out[-2000:])
+    assert ok is True and f"Validation Passed: count {KAT['count']}" in out
+    assert "iter_num:" in out and "dyn_offset_num:" in out
+    res = global_timer.inst_result
+    # 4 groups over 10,000 bytes: one tile each, at least one pass per tile
+    assert res["iter_num_total"] >= KAT["groups"]

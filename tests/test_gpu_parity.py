@@ -168,7 +168,8 @@ def test_tile_scope_replays(gpu, seed):
     n = rng.choice([9000, 70000, 300001])
     data = np.random.default_rng(300 + seed).choice(alpha, size=n, p=p)
     groups = [rng.sample(POOL, rng.randint(1, 4)) for _ in range(rng.randint(1, 4))]
-    opts = {"replay_scope": "tile", "groups_per_kernel": 1 + seed % 3, "bigram_filter": seed % 2}
+    opts = {"replay_scope": "tile", "carry_settle": "inline", "groups_per_kernel": 1 + seed % 3,
+            "bigram_filter": seed % 2}
     print(f"seed={seed} n={n} opts={opts} groups={groups}", flush=True)
     assert_parity(groups, data, gpu, options=opts, repeat=2, label=f"tile scope seed {seed}")
 

@@ -69,11 +69,13 @@ def _wave_add(a, b, l0, cin, pass_lane):
     return s, c
 
 
-def run_tiles(prog, basis):
+def run_tiles(prog, basis, stats=None):
     """Streams [n_out, W] computed tile by tile as the generated kernel does.
     ``prog``: a GroupProgram (its final instruction list and geometry);
     ``basis``: uint64 [streams, W] -- the 8 basis streams, or the workspace
-    of a stream-level step."""
+    of a stream-level step.  ``stats``: a list that receives, per tile, the
+    number of zero / byte-pair tests that fell through (what the kernel's
+    statistics word counts in the tile's exact pass)."""
     W = basis.shape[1]
     HL, HR, S = prog.HL, prog.HR, prog.S
     insts = prog.insts
@@ -98,6 +100,7 @@ def run_tiles(prog, basis):
                 return MASK if c == "1" else 0
             return env.get(v, 0)
 
+        passed = 0
         pc = 0
         while pc < len(insts):
             inst = insts[pc]
@@ -135,9 +138,13 @@ def run_tiles(prog, basis):
                     pairs = set(((b.astype(np.int64) & 0x7F) << 8 | nxt).tolist())
                 if not all(any(h in pairs for h in clause) for clause in inst.clauses):
                     pc = labels[inst.label] + 1
+                else:
+                    passed += 1
             elif ty == BsInstType.SKIPZERO:
                 if val(inst.operands[0]) == 0:
                     pc = labels[inst.label] + 1
+                else:
+                    passed += 1
             elif ty == BsInstType.STREAMSTORE:
                 outs[stores.index(pc - 1)] = val(inst.operands[0])
             elif ty in (BsInstType.LABEL, BsInstType.GRAPHBREAK):
@@ -145,6 +152,8 @@ def run_tiles(prog, basis):
             else:
                 raise NotImplementedError(f"tile model: {inst}")
         cin = cout
+        if stats is not None:
+            stats.append(passed)
         lo, hi = (0, WAVE - HR) if t == 0 else (HL, HL + S)
         for j, x in enumerate(outs):
             for lane in range(lo, hi):
