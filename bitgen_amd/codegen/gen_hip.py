@@ -154,7 +154,10 @@ class GroupProgram:
         at the end of its pass, waiting for its predecessor, and replays in
         a loop; "launch" -- the kernel runs twice, a sweep that never waits
         and a settle that replays only the tiles that need it (one pass per
-        tile and launch, so no pass loop).  inst_stats_dyn: each tile also
+        tile and launch, so no pass loop); "defer" -- one launch in which a
+        wave settles each tile after its next tile's pass, by when the
+        tile's predecessor has almost always published its statuses (a pass
+        that aborted is settled at once).  inst_stats_dyn: each tile also
         writes one statistics word into an extra status slot (index
         n_carry): passes run (bits 20..31) and zero / byte-pair tests that
         fell through in its exact pass (bits 0..19)."""
@@ -166,8 +169,8 @@ class GroupProgram:
         self.replay_scope = replay_scope
         if lookback not in ("speculate", "wait"):
             raise ValueError(f"lookback must be 'speculate' or 'wait', not {lookback!r}")
-        if carry_settle not in ("inline", "launch"):
-            raise ValueError(f"carry_settle must be 'inline' or 'launch', not {carry_settle!r}")
+        if carry_settle not in ("inline", "launch", "defer"):
+            raise ValueError(f"carry_settle must be 'inline', 'launch' or 'defer', not {carry_settle!r}")
         self.carry_settle = carry_settle
         self.inst_stats_dyn = bool(inst_stats_dyn)
         self.gid = gid
@@ -476,9 +479,17 @@ class GroupProgram:
         self.n_carry = len(self.carry_ops)
 
     @property
+    def phased(self):
+        """The tile program runs in phases: phase 0 a single speculative pass
+        that stores its values without settling its carry-ins, phase 1 the
+        settle (and the replay when one is needed).  Both the sweep + settle
+        launches and the deferred settle run the program this way."""
+        return 0 < self.n_carry <= 64 and self.lookback == "speculate" and self.carry_settle in ("launch", "defer")
+
+    @property
     def two_launch(self):
         """The kernel runs as a sweep launch and a settle launch."""
-        return 0 < self.n_carry <= 64 and self.lookback == "speculate" and self.carry_settle == "launch"
+        return self.phased and self.carry_settle == "launch"
 
     def n_tiles(self, n_words):
         s0 = WAVE - self.HR
@@ -506,7 +517,7 @@ class GroupProgram:
             words = self.bigram_key_words()
             keys = ", ".join(f"0x{k:08X}u" for k in words)
             add(f"static __constant__ uint32_t bg_keys_{self.gid}[{len(words)}] = {{{keys}}};")
-        add(f"static __device__ __forceinline__ void bg_prog_{self.gid}(const uint64_t *__restrict__ basis, "
+        add(f"static __device__ __forceinline__ bool bg_prog_{self.gid}(const uint64_t *__restrict__ basis, "
             "uint64_t *__restrict__ result, uint32_t *status, int64_t W, uint32_t t, uint32_t epoch, "
             "uint32_t n_carry, uint32_t *err, uint32_t *bgt, uint32_t phase) {")
         add(f"  constexpr int HL = {self.HL}, HR = {self.HR}, S = {self.S};")
@@ -515,12 +526,12 @@ class GroupProgram:
         add("  const int64_t w = w0 + lane;")
         add("  const bool inb = (w >= 0) && (w < W);")
         add("  bg_tile tc = bg_tile_make(t, w0, W, status, n_carry, epoch, err);")
-        two = self.two_launch
+        two = self.phased
         if two:
-            # settle launch: most tiles are exact from the sweep and stop here,
-            # before any load
+            # settle phase: most tiles are exact from their speculative pass
+            # and stop here, before any load
             add("  tc.sweep = phase == 0;")
-            add(f"  if (phase != 0 && bg_settle_begin<{self.n_carry}>(tc)) return;")
+            add(f"  if (phase != 0 && bg_settle_begin<{self.n_carry}>(tc)) return true;")
         else:
             add("  (void)phase;")
         if self.n_carry > 64 or self.lookback == "wait":
@@ -751,13 +762,14 @@ class GroupProgram:
                 add("    *bg_dw = bg_dyn;")
             add("  }")
         if two:
-            add("  if (!bg_keep) return;")
+            add("  if (!bg_keep) return false;")
         add("  const int lo = (t == 0) ? 0 : HL;")
         add("  const int hi = (t == 0) ? 64 - HR : HL + S;")
         add("  if (inb && lane >= lo && lane < hi) {")
         for j, o in enumerate(outs):
             add(f"    result[{f'(int64_t){j} * W + ' if j else ''}w] = {o};")
         add("  }")
+        add("  return true;")
         add("}")
         return "\n".join(lines) + "\n"
 
@@ -995,8 +1007,8 @@ class GroupProgram:
 
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
-        call = (f"bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, a.n_carry, "
-                "a.ctl + BG_CTL_ERROR, bgt, phase);")
+        call = (f"bg_ok = bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, "
+                "a.n_carry, a.ctl + BG_CTL_ERROR, bgt, phase);")
         chunk = [
             "const uint32_t tl = g + (uint32_t)bg_lane();",
             "uint32_t *row = tl < a.n_tiles ? a.status + (size_t)tl * a.n_carry : nullptr;",
@@ -1019,13 +1031,52 @@ class GroupProgram:
     def work_loop(self, n_tiles, run, chunk):
         """Kernel body: persistent waves take tiles from the control block's
         queues (the next take issued before the current tile runs) and run
-        each (`run`: statements for tile t, phase `phase`; one call site, as
-        the tile program is inlined).  Two-launch form, settle launch
-        (phase 1): waves take chunks of 64 consecutive tiles from 8 queues,
-        resolve a chunk's carry-ins at once (`chunk`: statements setting
-        `need` for tiles g .. g + 63) and run only the tiles in `need`, in
-        order."""
+        each (`run`: statements for tile t, phase `phase`, that set `bg_ok`
+        -- false when a phase-0 pass aborted; one call site, as the tile
+        program is inlined).  Two-launch form, settle launch (phase 1): waves
+        take chunks of 64 consecutive tiles from 8 queues, resolve a chunk's
+        carry-ins at once (`chunk`: statements setting `need` for tiles g ..
+        g + 63) and run only the tiles in `need`, in order.  Deferred
+        settle: each iteration runs either a new tile's speculative pass
+        (phase 0) or the settle of a tile passed earlier (phase 1): a tile is
+        settled after the wave's next pass, or at once when its own pass
+        aborted (its later statuses are not published until it replays, so
+        it must not wait behind another pass), and the last one after the
+        queue runs dry."""
         ind = lambda xs, n: [" " * n + x for x in xs]
+        if self.phased and self.carry_settle == "defer":
+            none = "0xFFFFFFFFu"
+            return [
+                f"  const bg_queue Q = bg_queue_open(a.ctl, a.n_queues, {n_tiles}, 1);",
+                "  (void)phase;",
+                "  uint32_t cur, next = 0;",
+                "  bool more = bg_queue_take(Q, cur);",
+                f"  uint32_t pend = {none}, now = {none};  // passed, not yet settled; to settle next",
+                "  while (true) {",
+                "    uint32_t t, ph;",
+                f"    if (now != {none}) {{ t = now; ph = 1; now = {none}; }}",
+                "    else if (more) { t = cur; ph = 0; next = bg_queue_issue(Q); }",
+                f"    else if (pend != {none}) {{ t = pend; ph = 1; pend = {none}; }}",
+                "    else break;",
+                "    // the tile and the pass outcome are wave-uniform: said explicitly, as",
+                "    // the compiler cannot see it through the loop (it would otherwise keep",
+                "    // the tile's addresses in vector registers)",
+                "    t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);",
+                "    bool bg_ok;",
+                "    {",
+                "      const uint32_t phase = ph;",
+                *ind(run, 6),
+                "      bg_ok = __builtin_amdgcn_readfirstlane((int)bg_ok) != 0;",
+                "    }",
+                "    if (ph == 0) {",
+                "      more = bg_queue_finish(Q, next, cur);",
+                "      if (!bg_ok) now = t;",
+                "      else { now = pend; pend = t; }",
+                "    }",
+                "  }",
+                "  bg_queue_close(Q);",
+            ]
+        run = ["{", "  bool bg_ok;", *ind(run, 2), "  (void)bg_ok;", "}"]
         if not self.two_launch:
             return [
                 "  // persistent: each wave takes tiles from its block's queue until the",
@@ -1115,8 +1166,8 @@ def batch_source(program, block_threads=256):
     run = [
         "const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[t]);",
         "const bg_job_t J = a.jobs[j];",
-        f"bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, t - J.tile_start, Q.epoch, J.n_carry, "
-        "a.ctl + BG_CTL_ERROR, bgt, phase);",
+        f"bg_ok = bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, t - J.tile_start, Q.epoch, "
+        "J.n_carry, a.ctl + BG_CTL_ERROR, bgt, phase);",
     ]
     chunk = [
         "// lane i: global tile g + i, its job and row; lane 0's job for the",

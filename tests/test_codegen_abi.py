@@ -112,10 +112,10 @@ def test_two_launch_form_has_one_pass_per_launch():
     src = p.source()
     assert p.two_launch and p.loop_start is None and p.meta()["launches"] == 2
     assert "for (;;)" not in src and "bg_settle<" not in src
-    begin = src.index(f"if (phase != 0 && bg_settle_begin<{p.n_carry}>(tc)) return;")
+    begin = src.index(f"if (phase != 0 && bg_settle_begin<{p.n_carry}>(tc)) return true;")
     assert begin < src.index("bas[i] = basis[")
     assert f"const bool bg_keep = bg_sweep_end<{p.n_carry}>(tc);" in src
-    assert src.index("if (!bg_keep) return;") < src.index("result[w] = out0;")
+    assert src.index("if (!bg_keep) return false;") < src.index("result[w] = out0;")
     assert src.count("goto bg_pass_end;") == p.n_carry
     assert "bg_queue_close(Q, phase != 0);" in src
     assert 'extern "C" __global__ void bg_group_0_two_launch() {}' in src
@@ -126,6 +126,34 @@ def test_two_launch_form_has_one_pass_per_launch():
     assert not w.two_launch and "bg_queue_close(Q);" in w.source() and "_two_launch" not in w.source()
     with pytest.raises(ValueError):
         _gprog(["x.*y"], carry_settle="later")
+
+
+def test_deferred_settle_form():
+    """carry_settle="defer": the tile program of the two-launch form (phase 0
+    a single speculative pass, phase 1 bg_settle_begin + replay) driven by
+    one launch: each iteration makes the one inlined call, on a new tile
+    (phase 0) or on a tile passed before (phase 1); a tile is settled after
+    the wave's next pass, an aborted pass at once, the last pending tile
+    after the queue runs dry; no marker kernel, one launch; the tile index
+    and the pass outcome are made wave-uniform."""
+    from bitgen_amd.codegen.gen_hip import batch_source
+    p = _gprog(["ab[^\\n]*cd", "x.*y", "q[0-9]+z"], carry_settle="defer")
+    src = p.source()
+    assert p.phased and not p.two_launch and p.loop_start is None and p.meta()["launches"] == 1
+    assert "for (;;)" not in src and "bg_settle<" not in src and "_two_launch" not in src
+    assert src.count("bg_ok = bg_prog_0(") == 1
+    assert f"if (phase != 0 && bg_settle_begin<{p.n_carry}>(tc)) return true;" in src
+    loop = src[src.index("__global__"):]
+    assert "if (!bg_ok) now = t;" in loop and "else { now = pend; pend = t; }" in loop
+    assert loop.index("if (now != 0xFFFFFFFFu)") < loop.index("else if (more)") < loop.index("else if (pend != 0xFFFFFFFFu)")
+    assert "t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);" in loop
+    assert "bg_ok = __builtin_amdgcn_readfirstlane((int)bg_ok) != 0;" in loop
+    assert "bg_queue_close(Q);" in loop and "bg_queue_open(a.ctl, a.n_queues, a.n_tiles, 1)" in loop
+    b = batch_source(p)
+    assert b.count("bg_ok = bg_prog_0(") == 1 and "bg_queue_open(a.ctl, a.n_queues, a.total_tiles, 1)" in b
+    # programs without speculation stay in the plain loop
+    w = _gprog(["x.*y"], carry_settle="defer", lookback="wait")
+    assert not w.phased and "pend" not in w.source()
 
 
 def test_carry_bounds_follow_data_flow():
@@ -509,7 +537,7 @@ def test_inst_stats_dyn_slot_and_counting():
     counts one, each pass adds 1 << 20; the word is written by lane 0 before
     the stores, in both settle forms; without the option nothing changes."""
     regs = ["ab[^\\n]*cd", "x.*y", "hello", "q[0-9]+z"]
-    for settle in ("inline", "launch"):
+    for settle in ("inline", "launch", "defer"):
         p = _gprog(regs, carry_settle=settle, inst_stats_dyn=True, bigram_filter=True)
         src = p.source()
         m = p.meta()
@@ -523,7 +551,7 @@ def test_inst_stats_dyn_slot_and_counting():
         if settle == "inline":
             assert "bg_dyn = (bg_dyn & 0xFFF00000u) + (1u << 20);" in src  # each pass of the loop
         else:
-            assert "uint32_t bg_dyn = 1u << 20;" in src and "if (!bg_keep) return;" in src
+            assert "uint32_t bg_dyn = 1u << 20;" in src and "if (!bg_keep) return false;" in src
     q = _gprog(regs)
     assert "bg_dyn" not in q.source() and q.meta()["status_slots"] == max(1, q.n_carry)
     assert q.meta()["dyn_stats"] is False

@@ -106,12 +106,14 @@ def test_batched_bigram_filter(gpu):
 
 
 
-def test_batched_two_launch_graph(gpu):
-    """Batch kernels in the sweep + settle form: both launches of every
-    kernel are captured into the graph; eager and every replay bit-exact."""
+@pytest.mark.parametrize("settle", ["launch", "defer"])
+def test_batched_phased_graph(gpu, settle):
+    """Batch kernels in the sweep + settle form (both launches of every
+    kernel captured into the graph) and in the deferred-settle form (one
+    launch); eager and every replay bit-exact."""
     import torch
 
-    sets = [BatchPatternSet(s, {"carry_settle": "launch", "groups_per_kernel": k}).load(0)
+    sets = [BatchPatternSet(s, {"carry_settle": settle, "groups_per_kernel": k}).load(0)
             for s, k in zip(SETS, (1, 2))]
     buffers = _buffers(7, 12)
     jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers))]

@@ -119,7 +119,7 @@ def test_plain_small_workload(gpu):
     assert O.popcount(ref) > 0
 
 
-@pytest.mark.parametrize("settle", ["inline", "launch"])
+@pytest.mark.parametrize("settle", ["inline", "launch", "defer"])
 def test_carry_crosses_many_tiles(gpu, settle):
     # one 300 KB line: MatchStar/ScanThru carries and the line-end projection
     # propagate through ~75 tiles
@@ -135,7 +135,8 @@ def test_carry_crosses_many_tiles(gpu, settle):
 
 CARRY_FORMS = {"speculate": {"lookback": "speculate", "carry_settle": "inline"},
                "wait": {"lookback": "wait"},
-               "launch": {"lookback": "speculate", "carry_settle": "launch"}}
+               "launch": {"lookback": "speculate", "carry_settle": "launch"},
+               "defer": {"lookback": "speculate", "carry_settle": "defer"}}
 
 
 @pytest.mark.parametrize("lookback", list(CARRY_FORMS))
@@ -210,14 +211,17 @@ def test_more_than_64_carry_ops(gpu):
     assert_parity([regs], data, gpu, label="70 stars")
 
 
+@pytest.mark.parametrize("settle", ["launch", "defer"])
 @pytest.mark.parametrize("seed", range(10))
-def test_two_launch_settle(gpu, seed):
+def test_phased_settle(gpu, seed, settle):
     """carry_settle="launch": a sweep launch that never waits at the end of a
     tile, then a settle launch that replays only tiles whose assumed
-    carry-ins were wrong or whose sweep aborted (deferred statuses).  Star-
-    and loop-heavy groups, short and long lines, single, merged and filtered
-    kernels, stream-level groups; each run twice (the second run's epoch
-    must not see the first run's statuses)."""
+    carry-ins were wrong or whose sweep aborted (deferred statuses);
+    carry_settle="defer": the same phases in one launch, each tile settled
+    after its wave's next pass.  Star- and loop-heavy groups, short and long
+    lines, single, merged and filtered kernels, stream-level groups; each
+    run twice (the second run's epoch must not see the first run's
+    statuses)."""
     rng = random.Random(400 + seed)
     alpha = np.frombuffer(b"abcdxyzq0123\n", dtype=np.uint8)
     p = [0.96 / 12] * 12 + [0.04] if seed % 2 else [1 / 13] * 13
@@ -230,7 +234,7 @@ def test_two_launch_settle(gpu, seed):
         data[off:off + seg.size] = seg
     pool = POOL if seed % 3 == 0 else FUSED_POOL
     groups = [rng.sample(pool, rng.randint(1, 4)) for _ in range(rng.randint(1, 6))]
-    opts = {"carry_settle": "launch", "groups_per_kernel": 1 + seed % 4 if pool is FUSED_POOL else 1,
+    opts = {"carry_settle": settle, "groups_per_kernel": 1 + seed % 4 if pool is FUSED_POOL else 1,
             "bigram_filter": seed % 2 if pool is FUSED_POOL else 0, "min_waves": 5 if seed % 4 == 3 else 0}
     print(f"seed={seed} n={n} opts={opts} groups={groups}", flush=True)
-    assert_parity(groups, data, gpu, options=opts, repeat=2, label=f"two-launch seed {seed}")
+    assert_parity(groups, data, gpu, options=opts, repeat=2, label=f"{settle} seed {seed}")

@@ -1,7 +1,7 @@
 """Per-tile dynamic statistics (inst_stats_dyn, the reference's
 --pass-inst-stats-dyn) on the GPU: the count of zero / byte-pair tests that
 fell through in each tile's exact pass equals the CPU tile model's count for
-the same program and bytes, in both carry settle forms; every tile ran at
+the same program and bytes, in every carry settle form; every tile ran at
 least one pass; results stay bit-exact."""
 import random
 
@@ -19,7 +19,7 @@ POOL = ["ab[^\\n]*cd", "x.*y", "hello", "q[0-9]+z", "zz[0-9]{2,4}y", "a.{0,3}b",
         "bb[^a]*c", "xyz$", "^abc"]
 
 
-@pytest.mark.parametrize("settle", ["inline", "launch"])
+@pytest.mark.parametrize("settle", ["inline", "launch", "defer"])
 @pytest.mark.parametrize("seed", range(3))
 def test_dyn_stats_match_tile_model(gpu, settle, seed):
     import torch
