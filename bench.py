@@ -495,26 +495,33 @@ def served_measurement(args):
     return out
 
 
-def profiled_traffic(args, options):
-    """HBM bytes per group-kernel launch measured by rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE, scaled by a plain copy kernel) for the default workload, from
-    the committed counter summary; None for other workloads or another launch
-    shape than the one profiled."""
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD
+# every 4 cycles at 2.4 GHz (giga wave-instructions per second)
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
+
+
+def profiled_counters(args, options):
+    """Counters of the group kernels measured by rocprofv3 for the default
+    workload, from the committed counter summary: {"traffic": HBM bytes per
+    launch (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel),
+    "valu": [SQ_INSTS_VALU per launch, per kernel]}; None for other workloads
+    or another launch shape than the one profiled."""
     if (args.mib, args.regexes, args.groups) != (100, 1000, 16):
         return None
     path = os.path.join(ROOT, "profiles", "bg_group_ids_100MiB_counters.md")
     if not os.path.exists(path):
         return None
-    traffic, shape = None, None
+    found, shape = {}, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch"):
-            traffic = int(float(line.split("=")[1]))
+            found["traffic"] = int(float(line.split("=")[1]))
+        elif line.startswith("valu_insts_per_launch"):
+            found["valu"] = json.loads(line.split("=", 1)[1])
         elif line.startswith("shape ="):
             shape = dict(kv.split("=") for kv in line.split("=", 1)[1].split())
-    want = {"groups_per_kernel": str(options.get("groups_per_kernel", 1)), "min_waves": str(options.get("min_waves", 0)),
-            "bigram_filter": str(options.get("bigram_filter", 0))}
-    shape = shape and {"bigram_filter": "0", **shape}
-    return traffic if shape == want else None
+    keys = ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads", "carry_settle")
+    want = {k: str(options.get(k)) for k in keys}
+    return found if shape == want else None
 
 
 def main():
@@ -619,7 +626,9 @@ def main():
     # algorithmic bytes of a kernel launch: basis read (N bytes) + its groups'
     # match streams written (8 W bytes each)
     alg_bytes = [n_bytes + m.get("n_out", 1) * W * 8 for m in ks.metas]
-    achieved = sum(alg_bytes) / (float(np.mean([sum(r) for r in kernel_ms])) / 1e3) / 1e9
+    mean_kernels_s = float(np.mean([sum(r) for r in kernel_ms])) / 1e3
+    achieved = sum(alg_bytes) / mean_kernels_s / 1e9
+    counters = profiled_counters(args, cs.options)
     per_kernel = [float(np.median([kernel_ms[s][k] for s in range(args.steps)])) for k in range(K)]
 
     count = None
@@ -665,8 +674,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": profiled_traffic(args, cs.options),
+            "traffic": counters.get("traffic") if counters else None,
         },
+        # the same launches against the VALU issue peak: SQ_INSTS_VALU per
+        # launch (committed counter profile of this launch shape) / the
+        # launches' measured time
+        "roofline_valu": ({
+            "bound": "valu",
+            "achieved": round(sum(counters["valu"]) / 1e9 / mean_kernels_s, 3),
+            "peak": VALU_PEAK_GINST,
+            "unit": "G wave-instructions/s",
+            "frac": round(sum(counters["valu"]) / 1e9 / mean_kernels_s / VALU_PEAK_GINST, 6),
+        } if counters and "valu" in counters and len(counters["valu"]) == K else None),
     }
     if count is not None:
         out["match_count"] = count
