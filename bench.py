@@ -111,9 +111,8 @@ def prepare(args):
     tasks = [(compiler.compile_pattern_set, g, autotune.options_of(c))
              for g in (ids_groups, gen_groups) for c in autotune.candidates(len(g))]
     # the batched and served sets take the IDS set's tuned shape: prepare the
-    # usual one at every block size the tuner may pick
-    usual = autotune.USUAL
-    shapes = [autotune.options_of(usual[:4] + (bt,)) for bt in sorted({c[4] for c in autotune.CANDIDATES})]
+    # usual one (another choice compiles when first loaded)
+    shapes = [autotune.options_of(autotune.USUAL)]
     for regs in workloads.batched_sets() + workloads.served_sets():
         g = compiler.group_regex_in_list(regs, workloads.groups_for(len(regs)))
         for opts in shapes:

@@ -29,3 +29,16 @@ def test_cache_key_depends_on_set_sample_device_and_code():
     assert k != autotune._key(g, None, 1 << 20, "other")
     assert k != autotune._key(g[:1], None, 1 << 20, "dev")
     assert k != autotune._key(g, {"bigram_filter": 1}, 1 << 20, "dev")
+
+
+def test_pick_breaks_near_ties_by_candidate_order():
+    rows = [{"shape": c, "ms": ms} for c, ms in zip("abcd", (0.70, 0.526, 0.527, 0.5237))]
+    # b, c, d are within 2% of the fastest (d): b is listed first
+    assert autotune.pick(rows)["shape"] == "b"
+    rows[1]["ms"] = 0.60
+    assert autotune.pick(rows)["shape"] == "c"
+    rows[2]["ms"] = 0.60
+    assert autotune.pick(rows)["shape"] == "d"
+    # the usual shape is listed before the same shape at other block sizes
+    order = [c for c in autotune.CANDIDATES if c[:4] == autotune.USUAL[:4]]
+    assert order[0] == autotune.USUAL
