@@ -120,8 +120,11 @@ def _members(m):
 
 
 def bigram_hash(a, b):
-    """Same function as bg_bigram_hash in the device header."""
-    return ((a & 0x7F) << 8) | (b & 0xFF)
+    """Same function as bg_bigram_hash in the device header: table word
+    (hash >> 5) = the low 2 bits of a and all of b, bit in the word (hash &
+    31) = bits 2..6 of a.  The word's low bits, which pick the LDS bank, are
+    b's low bits (spread evenly in text; its top bits take a few values)."""
+    return ((((a & 3) << 8) | (b & 0xFF)) << 5) | ((a >> 2) & 31)
 
 
 class GroupProgram:

@@ -81,6 +81,10 @@ BG_HD void bg_untranspose_block(const uint64_t y[8], uint64_t x_out[8]) {
 
 // Bigram filter hash: 15 bits of the byte pair (a, b) at positions (p, p+1):
 // the low 7 bits of a and all of b.  Pairs that differ only in bit 7 of a
-// share a slot (a superset filter, never a missed pair).
-BG_HD uint32_t bg_bigram_hash(uint32_t a, uint32_t b) { return ((a & 0x7Fu) << 8) | (b & 0xFFu); }
+// share a slot (a superset filter, never a missed pair).  Table word (hash >>
+// 5) = (a & 3) << 8 | b, bit (hash & 31) = bits 2..6 of a: the word's low
+// bits select the LDS bank, and b's low bits spread evenly in text.
+BG_HD uint32_t bg_bigram_hash(uint32_t a, uint32_t b) {
+  return ((((a & 3u) << 8) | (b & 0xFFu)) << 5) | ((a >> 2) & 31u);
+}
 

@@ -169,12 +169,13 @@ __device__ __forceinline__ void bg_bigram_build(const uint64_t bas[8], uint32_t 
                             __builtin_amdgcn_perm(after, hi, 0x03040203u)};
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      // word = hash >> 5 = bits 5..14 of the 16-bit key (the first byte's top
-      // bit, bit 15, is not part of the hash), bit = hash & 31
+      // word = hash >> 5 = bits 0..9 of the 16-bit key (b and the low two
+      // bits of a), bit = hash & 31 = bits 10..14 (a's bits 2..6; the first
+      // byte's top bit, bit 15, is not part of the hash)
       const uint32_t k = kk[q];
-      __hip_atomic_fetch_or(tab + __builtin_amdgcn_ubfe(k, 5u, 10u), 1u << (k & 31u), __ATOMIC_RELAXED,
+      __hip_atomic_fetch_or(tab + (k & 0x3FFu), 1u << ((k >> 10) & 31u), __ATOMIC_RELAXED,
                             __HIP_MEMORY_SCOPE_WAVEFRONT);
-      __hip_atomic_fetch_or(tab + __builtin_amdgcn_ubfe(k, 21u, 10u), 1u << ((k >> 16) & 31u), __ATOMIC_RELAXED,
+      __hip_atomic_fetch_or(tab + __builtin_amdgcn_ubfe(k, 16u, 10u), 1u << ((k >> 26) & 31u), __ATOMIC_RELAXED,
                             __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
   }

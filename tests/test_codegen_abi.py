@@ -290,7 +290,8 @@ int main() {
 
 def test_untranspose_core_host_build(tmp_path):
     """The inverse block transpose (used by the bigram filter) gives back the
-    bytes for every block, and bg_bigram_hash keeps b and the low 7 bits of a."""
+    bytes for every block, and bg_bigram_hash keeps b and the low 7 bits of a
+    (word = (a & 3) << 8 | b, bit = bits 2..6 of a)."""
     src = tmp_path / "u.cpp"
     src.write_text(r'''
 #include <cstdio>
@@ -307,7 +308,7 @@ int main() {
   }
   for (unsigned a = 0; a < 256; a++)
     for (unsigned b = 0; b < 256; b++)
-      bad += bg_bigram_hash(a, b) != (((a & 127u) << 8) | b);
+      bad += bg_bigram_hash(a, b) != (((((a & 3u) << 8) | b) << 5) | ((a >> 2) & 31u));
   printf("%d\n", bad); return bad != 0;
 }''')
     exe = tmp_path / "u"
@@ -518,17 +519,24 @@ def test_bigram_filter_keeps_carry_ops_in_every_tile():
 
 
 def test_bigram_table_slot_matches_hash():
-    """The table insert takes word = bits 5..14 and bit = bits 0..4 of the
-    16-bit key (first byte << 8 | second byte); that is the 15-bit pair hash
-    the filters are keyed on, for every byte pair (the first byte's top bit
-    is not part of the hash)."""
+    """The table insert takes word = bits 0..9 and bit = bits 10..14 of the
+    16-bit key (first byte << 8 | second byte), for both keys of a 32-bit
+    v_perm result; that is the 15-bit pair hash the filters are keyed on,
+    for every byte pair (the first byte's top bit is not part of the hash),
+    and it is one-to-one on the 15 bits it keeps."""
     from bitgen_amd.codegen.gen_hip import bigram_hash
 
+    seen = set()
     for a in range(256):
         for b in range(256):
             key = (a << 8) | b
+            k32 = (key << 16) | 0x1234
             h = bigram_hash(a, b)
-            assert (key >> 5) & 0x3FF == h >> 5 and key & 31 == h & 31
+            assert key & 0x3FF == h >> 5 and (key >> 10) & 31 == h & 31
+            assert (k32 >> 16) & 0x3FF == h >> 5 and (k32 >> 26) & 31 == h & 31
+            assert h < 1 << 15
+            seen.add(h)
+    assert len(seen) == 1 << 15
 
 
 def test_inst_stats_dyn_slot_and_counting():

@@ -135,7 +135,8 @@ def run_tiles(prog, basis, stats=None):
                 if pairs is None:
                     b = _window_bytes([bas[i] for i in range(8)])
                     nxt = np.append(b[1:], 0)
-                    pairs = set(((b.astype(np.int64) & 0x7F) << 8 | nxt).tolist())
+                    a64, n64 = b.astype(np.int64), nxt.astype(np.int64)
+                    pairs = set(((((a64 & 3) << 8 | n64) << 5) | ((a64 >> 2) & 31)).tolist())
                 if not all(any(h in pairs for h in clause) for clause in inst.clauses):
                     pc = labels[inst.label] + 1
                 else:
