@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--groups", type=int, default=16)
     ap.add_argument("--queues", type=int, default=0, help="tile queues per launch (0 = runtime default)")
     ap.add_argument("--grid-blocks", type=int, default=0, help="persistent blocks per launch (0 = all that fit)")
+    ap.add_argument("--filter-grid-blocks", type=int, default=0,
+                    help="blocks of the byte-pair filter pre-pass (0 = all that fit at once)")
     ap.add_argument("--take", type=int, default=0, help="tiles per queue take (0 = runtime default)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="compiler option override (e.g. pass_short_circuit_interval=2); skips the autotuner")
@@ -112,11 +114,14 @@ def prepare(args):
              for g in (ids_groups, gen_groups) for c in autotune.candidates(len(g))]
     # the batched and served sets take the IDS set's tuned shape: prepare the
     # usual one (another choice compiles when first loaded)
-    shapes = [autotune.options_of(autotune.USUAL)]
-    for regs in workloads.batched_sets() + workloads.served_sets():
+    opts = autotune.options_of(autotune.USUAL)
+    for regs in workloads.batched_sets():
         g = compiler.group_regex_in_list(regs, workloads.groups_for(len(regs)))
-        for opts in shapes:
-            tasks += [(compiler.compile_pattern_set, g, opts), (BatchPatternSet, g, opts)]
+        tasks += [(compiler.compile_pattern_set, g, opts), (BatchPatternSet, g, opts)]
+    for regs in workloads.served_sets():
+        # served sets build their byte-pair tables in the kernels
+        g = compiler.group_regex_in_list(regs, workloads.groups_for(len(regs)))
+        tasks += [(compiler.compile_pattern_set, g, opts), (BatchPatternSet, g, dict(opts, shared_filter=0))]
     with concurrent.futures.ThreadPoolExecutor(max_workers=3) as ex:
         for f in [ex.submit(fn, g, o) for fn, g, o in tasks]:
             f.result()
@@ -359,6 +364,7 @@ def batched_measurement(args):
             W = words_for(n)
             rt.transpose(staged[bi].data_ptr(), n, staged[bi].numel(), basis[bi].data_ptr(), W, 1, sptr)
             r, st = results[(si, bi)], status[(si, bi)]
+            ks.launch_filter(basis[bi].data_ptr(), W, sptr)
             for k in range(ks.n_kernels):
                 ks.launch(k, basis[bi].data_ptr(), r.data_ptr() + ks.first_group[k] * W * 8,
                           st.data_ptr() + k * st.size(2) * 4, W, sptr)
@@ -392,7 +398,8 @@ def batched_measurement(args):
         "batched_pct_hbm_peak": round(100 * scanned / (batched_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "plain_ms": round(plain_ms, 4),
         "plain_gbs": round(scanned / (plain_ms / 1e3) / 1e9, 3),
-        "plain_launches": len(jobs) + sum(psets[si].n_kernels for si, _ in jobs),
+        "plain_launches": len(jobs) + sum(psets[si].n_kernels + (psets[si].filter_kernel is not None)
+                                          for si, _ in jobs),
         "launch_shape": {k: opts.get(k, compiler.DEFAULT_OPTIONS[k])
                          for k in ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads")},
         "speedup": round(plain_ms / batched_ms, 3),
@@ -421,7 +428,9 @@ def served_measurement(args):
     opts = dict(getattr(args, "tuned_options", {}) or {})
     bsets, psets = {}, {}
     for i, g in enumerate(set_groups):
-        bsets[i] = BatchPatternSet(g, opts).load(0)
+        # the server's native staging launches kernels that build their own
+        # byte-pair tables
+        bsets[i] = BatchPatternSet(g, dict(opts, shared_filter=0)).load(0)
         psets[i] = compiler.compile_pattern_set(g, opts).load(0)
         _log(f"served: set {i} ({len(g)} groups) loaded ({time.perf_counter() - t0:.1f}s)")
     _log(f"served pattern sets loaded: {[len(g) for g in set_groups]} groups ({time.perf_counter() - t0:.1f}s)")
@@ -518,7 +527,7 @@ def profiled_counters(args, options):
             found["valu"] = json.loads(line.split("=", 1)[1])
         elif line.startswith("shape ="):
             shape = dict(kv.split("=") for kv in line.split("=", 1)[1].split())
-    keys = ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads", "carry_settle")
+    keys = ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads", "carry_settle", "shared_filter")
     want = {k: str(options.get(k)) for k in keys}
     return found if shape == want else None
 
@@ -570,6 +579,7 @@ def main():
     ks = cs.load(0)
     ks.n_queues = args.queues
     ks.grid_blocks = args.grid_blocks
+    ks.filter_grid_blocks = args.filter_grid_blocks
     ks.take = args.take
     from bitgen_amd.backend.hip import runtime as rt_
 
@@ -596,11 +606,14 @@ def main():
         transpose_bytes(staged, n_bytes, out=basis)
         if events is not None:
             events[1].record(stream)
+        ks.launch_filter(basis.data_ptr(), W, sptr)  # byte-pair answers of a set with shared filters
+        if events is not None:
+            events[2].record(stream)
         for k in range(K):
             ks.launch(k, basis.data_ptr(), result.data_ptr() + ks.first_group[k] * W * 8,
                       status.data_ptr() + k * status.size(2) * 4, W, sptr)
             if events is not None:
-                events[2 + k].record(stream)
+                events[3 + k].record(stream)
 
     for _ in range(args.warmup):
         step()
@@ -608,7 +621,7 @@ def main():
     ks.check_errors()
     print("[bench] warm-up done", file=sys.stderr, flush=True)
 
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(2 + K)] for _ in range(args.steps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3 + K)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for s in range(args.steps):
@@ -618,8 +631,9 @@ def main():
     ks.check_errors()
 
     trans_ms = [e[0].elapsed_time(e[1]) for e in ev]
-    kernel_ms = [[ev[s][1 + k].elapsed_time(ev[s][2 + k]) for k in range(K)] for s in range(args.steps)]
-    step_ms = [e[0].elapsed_time(e[1 + K]) for e in ev]
+    filter_ms = [e[1].elapsed_time(e[2]) for e in ev]
+    kernel_ms = [[ev[s][2 + k].elapsed_time(ev[s][3 + k]) for k in range(K)] for s in range(args.steps)]
+    step_ms = [e[0].elapsed_time(e[2 + K]) for e in ev]
     ms_per_step = float(np.median(step_ms))
     value = n_bytes / (ms_per_step / 1e3) / 1e9
     # algorithmic bytes of a kernel launch: basis read (N bytes) + its groups'
@@ -661,10 +675,14 @@ def main():
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 4),
         "transpose_ms": round(float(np.median(trans_ms)), 4),
+        # the set's byte-pair filter pre-pass (kernels with shared filters;
+        # 0 otherwise): part of the step, not of the group kernels' roofline
+        "filter_ms": round(float(np.median(filter_ms)), 4) if ks.filter_kernel is not None else 0.0,
         "kernel_ms": [round(v, 4) for v in per_kernel],
         "wall_s": round(wall, 3),
         "autotune": ({"chosen": {"groups_per_kernel": cs.options["groups_per_kernel"],
-                                 "min_waves": cs.options["min_waves"], "queues": args.queues},
+                                 "min_waves": cs.options["min_waves"], "queues": args.queues,
+                                 "shared_filter": cs.options.get("shared_filter", 0)},
                       "sample_bytes": tuning["sample_bytes"], "candidates": tuning["candidates"]}
                      if tuning else None),
         "roofline": {

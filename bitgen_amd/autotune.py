@@ -14,7 +14,9 @@ onto gfx950:
 * ``queues`` -- tile-queue counters the launch spreads its tile takes over;
 * ``bigram_filter`` -- per-tile byte-pair filters in front of the literal
   first stages of the regexes (a tile whose bytes hold none of a stage's
-  pairs skips that regex without evaluating its character classes);
+  pairs skips that regex without evaluating its character classes); 1:
+  each kernel builds the tile's pair table, 2: one filter pre-pass per set
+  builds it once and answers every kernel's filters (``shared_filter``);
 * ``block_threads`` -- waves per block (x 64 threads): the bytes a block
   scans per round of takes, and the granularity the CUs are filled in.
 
@@ -33,7 +35,7 @@ from . import config as cfg
 from . import compiler
 
 # candidate launch shapes: (groups_per_kernel, min_waves, queues,
-# bigram_filter, block_threads).  A tile is one wave whatever the block size,
+# bigram_filter (2 = shared by the set's kernels), block_threads).  A tile is one wave whatever the block size,
 # so block_threads sets the waves per block (and the bytes a block scans per
 # round: 4 KiB per wave); words per lane stay 1 (a 64-bit word per lane is the
 # wave64 shift and carry unit).
@@ -47,6 +49,8 @@ CANDIDATES = [
     (4, 5, 64, 1, 256),
     (4, 5, 64, 1, 128),
     (4, 5, 64, 1, 512),
+    (4, 5, 64, 2, 256),
+    (8, 5, 64, 2, 256),
 ]
 
 
@@ -75,7 +79,8 @@ def candidates(n_groups):
 def options_of(candidate, base=None):
     gpk, mw, _q, bg, bt = candidate
     o = dict(base or {})
-    o.update(groups_per_kernel=gpk, min_waves=mw, bigram_filter=bg, block_threads=bt)
+    o.update(groups_per_kernel=gpk, min_waves=mw, bigram_filter=int(bool(bg)), block_threads=bt,
+             shared_filter=int(bg == 2))
     return o
 
 
@@ -142,6 +147,7 @@ def tune(groups, sample, base_options=None, device=0, reps=5, use_cache=True):
         status = ks.alloc_status(1, W)
 
         def one_pass():
+            ks.launch_filter(basis.data_ptr(), W, sptr)
             for k in range(ks.n_kernels):
                 ks.launch(k, basis.data_ptr(), result.data_ptr() + ks.first_group[k] * W * 8,
                           status.data_ptr() + k * status.size(2) * 4, W, sptr)

@@ -27,6 +27,7 @@ class GroupArgs(ctypes.Structure):
         ("n_queues", ctypes.c_uint32),
         ("n_carry", ctypes.c_uint32),
         ("take", ctypes.c_uint32),
+        ("filter", ctypes.c_void_p),
     ]
 
 
@@ -56,6 +57,7 @@ class BatchArgs(ctypes.Structure):
         ("total_tiles", ctypes.c_uint32),
         ("n_queues", ctypes.c_uint32),
         ("take", ctypes.c_uint32),
+        ("filter", ctypes.c_void_p),
     ]
 
 
@@ -90,6 +92,7 @@ EXPORTS = (
     "bg_to_reference_layout", "bg_popcount", "bg_stream_sync", "bg_batch_launch",
     "bg_graph_begin", "bg_graph_end", "bg_graph_launch", "bg_graph_destroy", "bg_transpose_batch",
     "bg_kernel_occupancy", "bg_ctl_words", "bg_group_launch_static", "bg_serve_pack", "bg_serve_enqueue",
+    "bg_filter_launch", "bg_filter_batch_launch",
 )
 
 ENOSPACE = 1003
@@ -149,6 +152,8 @@ def load_library():
         L.bg_kernel_get.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
         L.bg_kernel_info.argtypes = [vp, ctypes.POINTER(KernelInfo)]
         L.bg_group_launch.argtypes = [vp, ctypes.POINTER(GroupArgs), u32, vp]
+        L.bg_filter_launch.argtypes = [vp, vp, u64, u32, vp, u32, vp]
+        L.bg_filter_batch_launch.argtypes = [vp, vp, vp, u32, vp, u32, vp]
         L.bg_kernel_occupancy.argtypes = [vp, u32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.bg_ctl_words.argtypes = []
         L.bg_to_reference_layout.argtypes = [vp, vp, u64, u64, vp]
@@ -237,6 +242,20 @@ def group_launch_static(kernel, basis_ptr, result_ptr, n_words, n_tiles, n_carry
     look-back error word lands in the device u32 at error_flag_ptr."""
     _check(load_library().bg_group_launch_static(kernel, basis_ptr, result_ptr, n_words, n_tiles, n_carry,
                                                  error_flag_ptr, stream), "bg_group_launch_static")
+
+
+def filter_launch(kernel, basis_ptr, n_words, n_tiles, filter_ptr, stream, grid_blocks=0):
+    """bg_filter_launch: a pattern set's byte-pair filter pre-pass over one
+    buffer (answers of every sharing kernel, [n_tiles][row] u32 words)."""
+    _check(load_library().bg_filter_launch(kernel, basis_ptr, n_words, n_tiles, filter_ptr, grid_blocks, stream),
+           "bg_filter_launch")
+
+
+def filter_batch_launch(kernel, jobs_ptr, tile_job_ptr, total_tiles, filter_ptr, stream, grid_blocks=0):
+    """bg_filter_batch_launch: the set's filter pre-pass over a batch's
+    global tiles (answers [total_tiles][row] u32 words)."""
+    _check(load_library().bg_filter_batch_launch(kernel, jobs_ptr, tile_job_ptr, total_tiles, filter_ptr,
+                                                 grid_blocks, stream), "bg_filter_batch_launch")
 
 
 def group_launch(kernel, args, stream, grid_blocks=0):

@@ -57,6 +57,21 @@ class HipGroupSet:
             else:
                 self.kernels.append(mod.kernel(m["kernel"]))
         self._workspaces = {}
+        # the set's byte-pair filter pre-pass (kernels compiled with shared
+        # filters): its kernel, a tile's answer words, one answer buffer per
+        # stream length
+        f = next((m["filter"] for m in metas if m.get("filter")), None)
+        self.filter_row = 0
+        self.filter_kernel = None
+        self.filter_grid_blocks = 0  # 0 = the pre-pass blocks that fit at once
+        self._filters = {}
+        if f is not None:
+            fpath = f["hsaco"] if os.path.isabs(f["hsaco"]) else os.path.join(base_dir, f["hsaco"])
+            fmod = rt.Module(fpath)
+            self.modules.append(fmod)
+            self.filter_kernel = fmod.kernel(f["kernel"])
+            self.filter_row = int(f["row"])
+            self.filter_meta = next(m for m in metas if m.get("filter_row"))
         # first group of each kernel (metas without "groups" hold one group)
         self.first_group = []
         g = 0
@@ -101,12 +116,39 @@ class HipGroupSet:
         return torch.zeros((n_inputs, self.n_kernels, self.status_words(n_words)), dtype=torch.int32,
                            device=torch.device("cuda", self.device))
 
+    def _filter_buffer(self, n_words):
+        torch = _torch()
+        buf = self._filters.get(n_words)
+        if buf is None:
+            n = self.n_tiles(self.filter_meta, n_words) * self.filter_row
+            buf = torch.empty(max(1, n), dtype=torch.int32, device=torch.device("cuda", self.device))
+            self._filters[n_words] = buf
+        return buf
+
+    def launch_filter(self, basis_ptr, n_words, stream):
+        """Enqueue the set's byte-pair filter pre-pass over one buffer, before
+        its group kernels on the same stream (no-op for a set without shared
+        filters).  The answers land in a buffer per stream length, so the
+        kernels of one buffer must follow before the next buffer's pre-pass
+        (stream order keeps that)."""
+        if self.filter_kernel is None:
+            return
+        buf = self._filter_buffer(n_words)
+        rt.filter_launch(self.filter_kernel, basis_ptr, n_words, self.n_tiles(self.filter_meta, n_words),
+                         buf.data_ptr(), stream, self.filter_grid_blocks)
+
     def launch(self, k, basis_ptr, result_ptr, status_ptr, n_words, stream):
         """Enqueue kernel k over one buffer (asynchronous).  ``result_ptr`` is
         the stream of the kernel's first group; its j-th group's stream
-        follows n_words words later for each j."""
+        follows n_words words later for each j.  A kernel with shared
+        byte-pair answers reads those of the last launch_filter of this
+        stream length."""
         meta = self.metas[k]
         args = rt.GroupArgs()
+        if meta.get("filter_row"):
+            if n_words not in self._filters:
+                raise rt.HipRuntimeError(f"kernel {k} reads shared byte-pair answers: call launch_filter first")
+            args.filter = self._filters[n_words].data_ptr()
         args.basis = basis_ptr
         args.result = result_ptr
         args.status = status_ptr
@@ -136,6 +178,7 @@ class HipGroupSet:
         s = rt.current_stream_ptr(stream)
         for m in range(M):
             bptr = basis.data_ptr() + m * 8 * W * 8
+            self.launch_filter(bptr, W, s)
             for k in range(K):
                 rptr = result.data_ptr() + (m * G + self.first_group[k]) * W * 8
                 sptr = status.data_ptr() + (m * K + k) * status.size(2) * 4

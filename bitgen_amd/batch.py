@@ -24,7 +24,7 @@ import os
 import numpy as np
 
 from .backend.hip import runtime as rt
-from .codegen.gen_hip import batch_kernel_name, batch_source, compile_sources
+from .codegen.gen_hip import FILTER_BATCH_KERNEL, batch_kernel_name, batch_source, compile_sources, filter_source
 from .compiler import DEFAULT_OPTIONS, kernel_meta, kernel_programs, load_cached_set, store_cached_set, _set_dir
 
 WAVE_WORDS = 64
@@ -56,6 +56,8 @@ class BatchPatternSet:
         self.block_threads = block_threads
         self.modules = None
         self.kernels = None
+        self.filter_module = None
+        self.filter_kernel = None
         d = _set_dir(self.groups, dict(o, form="batch", block=block_threads))
         cached = load_cached_set(d, "batch.json")
         if cached is not None:
@@ -74,10 +76,20 @@ class BatchPatternSet:
             with open(path, "w") as f:
                 f.write(src)
             sources[path] = src
+        # shared byte-pair answers: the set's filter pre-pass (its batched kernel)
+        row = max((p.filter_row for p in programs if getattr(p, "shared_filter", False)), default=0)
+        fpath = os.path.join(d, "filter.hip")
+        if row:
+            src = filter_source(programs)
+            with open(fpath, "w") as f:
+                f.write(src)
+            sources[fpath] = src
         hs = compile_sources(sources)
         self.hsacos = [os.path.abspath(hs[os.path.join(d, f"batch_{p.gid}.hip")]) for p in programs]
         for m, h in zip(self.metas, self.hsacos):
             m["hsaco"] = h
+            if row:
+                m["filter"] = {"hsaco": os.path.abspath(hs[fpath]), "kernel": FILTER_BATCH_KERNEL, "row": row}
         store_cached_set(d, "batch.json", self.metas)
 
     @property
@@ -93,7 +105,16 @@ class BatchPatternSet:
         if self.modules is None:
             self.modules = [rt.Module(h) for h in self.hsacos]
             self.kernels = [m.kernel(batch_kernel_name(k)) for k, m in enumerate(self.modules)]
+            f = next((m["filter"] for m in self.metas if m.get("filter")), None)
+            if f is not None:
+                self.filter_module = rt.Module(f["hsaco"])
+                self.filter_kernel = self.filter_module.kernel(f["kernel"])
         return self
+
+    @property
+    def filter_row(self):
+        """Answer words per tile of the set's shared byte-pair filters (0: none)."""
+        return max((m.get("filter_row", 0) for m in self.metas), default=0)
 
     def info(self):
         return [m.info(batch_kernel_name(k)) for k, m in enumerate(self.modules)]
@@ -104,6 +125,10 @@ class BatchPatternSet:
                 m.free()
             self.modules = None
             self.kernels = None
+        if self.filter_module is not None:
+            self.filter_module.free()
+            self.filter_module = None
+            self.filter_kernel = None
 
 
 class BatchPlan:
@@ -163,6 +188,20 @@ class BatchPlan:
                 self.queues.append(dict(set=si, kernel=kid, entries=entries, total_tiles=tile))
         self.result = torch.zeros(max(res_total, 1), dtype=torch.uint64, device=self.dev)
         self.status = torch.zeros(max(stat_total, 1), dtype=torch.int32, device=self.dev)
+        # sets with shared byte-pair filters: one answer buffer per set, rows
+        # by the global tiles of a sharing kernel's queue (every sharing
+        # kernel of the set has the same tiles per job)
+        self.filters = {}
+        for si, ps in enumerate(sets):
+            row = ps.filter_row
+            if not row:
+                continue
+            if ps.filter_kernel is None:
+                raise ValueError(f"set {si} shares its byte-pair filters: load() it first")
+            kid = next(k for k, m in enumerate(ps.metas) if m.get("filter_row"))
+            q = next(q for q in self.queues if q["set"] == si and q["kernel"] == kid)
+            buf = torch.empty(max(1, q["total_tiles"] * row), dtype=torch.int32, device=self.dev)
+            self.filters[si] = dict(buf=buf, queue=q)
         for q in self.queues:
             entries = q["entries"]
             arr = (rt.Job * max(1, len(entries)))()
@@ -189,6 +228,10 @@ class BatchPlan:
             args.n_queues = n_queues
             args.take = take
             q["args"] = args
+        for si, f in self.filters.items():
+            for q in self.queues:
+                if q["set"] == si and sets[si].metas[q["kernel"]].get("filter_row"):
+                    q["args"].filter = f["buf"].data_ptr()
         # one transpose launch for all buffers (a descriptor per buffer)
         td = (rt.TDesc * max(1, self.n_buffers))()
         ws = 0
@@ -218,8 +261,17 @@ class BatchPlan:
                            self.total_words, stream)
 
     def enqueue_match(self, stream):
+        done = set()
         for q in self.queues:
             ps = self.sets[q["set"]]
+            f = self.filters.get(q["set"])
+            if f is not None and q["set"] not in done:
+                # the set's byte-pair answers, before its kernels
+                done.add(q["set"])
+                fq = f["queue"]
+                if fq["total_tiles"]:
+                    rt.filter_batch_launch(ps.filter_kernel, fq["jobs"].data_ptr(), fq["tile_job"].data_ptr(),
+                                           fq["total_tiles"], f["buf"].data_ptr(), stream)
             if q["total_tiles"]:
                 rt.batch_launch(ps.kernels[q["kernel"]], q["args"], self._grid(ps, q["total_tiles"]),
                                 ps.block_threads, stream)

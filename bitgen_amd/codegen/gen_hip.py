@@ -132,7 +132,7 @@ class GroupProgram:
 
     def __init__(self, gid, insts, vnm, regexes=None, lookback="speculate", expect_zero=False, pipeline=False,
                  min_waves=0, bigram_filter=False, replay_scope="stage", block_threads=256, carry_settle="inline",
-                 inst_stats_dyn=False):
+                 inst_stats_dyn=False, shared_filter=False):
         """lookback: "speculate" (carry-ins assumed 0, resolved once per tile,
         replay when wrong) or "wait" (every carry op waits for its
         predecessor's status).  expect_zero: mark every zero test as likely
@@ -163,7 +163,11 @@ class GroupProgram:
         that aborted is settled at once).  inst_stats_dyn: each tile also
         writes one statistics word into an extra status slot (index
         n_carry): passes run (bits 20..31) and zero / byte-pair tests that
-        fell through in its exact pass (bits 0..19)."""
+        fell through in its exact pass (bits 0..19).  shared_filter (with
+        bigram_filter): the tile's byte-pair answers are read from a buffer
+        that one filter pre-pass of the whole pattern set fills (the
+        kernels of a set then share one tile geometry, see filter_source)
+        instead of each kernel building the tile's pair table itself."""
         if block_threads % WAVE or not 64 <= block_threads <= 1024:
             raise ValueError(f"block_threads must be a multiple of 64 in 64..1024, not {block_threads}")
         self.block_threads = int(block_threads)
@@ -198,6 +202,11 @@ class GroupProgram:
             self._bigram_rewrite()
         self._check_regions()
         self._halo()
+        self.shared_filter = bool(shared_filter) and bool(self.bigram_keys)
+        # word offset of this program's answers in a tile's row of the
+        # set's filter buffer, and the row length (set by plan_shared_filter)
+        self.filter_off = 0
+        self.filter_row = 0
 
     def _check_regions(self):
         """Carry ops never sit inside a skip range: every tile runs them, so
@@ -481,6 +490,25 @@ class GroupProgram:
                 self.carry_ops.append((inst, L))
         self.n_carry = len(self.carry_ops)
 
+    def set_geometry(self, HL, HR):
+        """Widen the tile's halo to HL / HR words (at least the program's own):
+        the kernels of a set that share a filter pre-pass cover the same
+        windows.  Each carry op's entry lane moves with HL (its operands stay
+        exact from the same distance behind it)."""
+        if HL < self.HL or HR < self.HR:
+            raise ValueError(f"group {self.gid}: halo {HL}/{HR} is narrower than the program's {self.HL}/{self.HR}")
+        if WAVE - HL - HR < 1:
+            raise HipCodegenError(f"group {self.gid}: a {HL}+{HR}-word halo leaves no tile")
+        self.HL, self.HR = HL, HR
+        self.S = WAVE - HL - HR
+        self.carry_ops = [(inst, HL - math.ceil(self.need.get(inst.ret, (0, 0))[0] / WAVE))
+                          for inst, _ in self.carry_ops]
+
+    @property
+    def filter_words(self):
+        """32-bit answer words of this program's byte-pair filters per tile."""
+        return 2 * ((len(self.bigram_keys) + 63) // 64)
+
     @property
     def phased(self):
         """The tile program runs in phases: phase 0 a single speculative pass
@@ -508,21 +536,24 @@ class GroupProgram:
             return "~0ull"
         return _cname(v)
 
-    def device_function(self):
+    def device_function(self, shared=None):
         """The group program for one tile as a device function (shared by the
-        plain per-group kernel and the batched kernel)."""
+        plain per-group kernel and the batched kernel).  shared: read the
+        byte-pair answers from the filter pre-pass (default: the program's
+        shared_filter)."""
+        shared = self.shared_filter if shared is None else bool(shared)
         lines = []
         add = lines.append
         regex_note = json.dumps(self.regexes)[:2000].replace("*/", "* /")
         add(f"/* group {self.gid}: {regex_note} */")
         nf = len(self.bigram_keys)
-        if nf:
+        if nf and not shared:
             words = self.bigram_key_words()
             keys = ", ".join(f"0x{k:08X}u" for k in words)
             add(f"static __constant__ uint32_t bg_keys_{self.gid}[{len(words)}] = {{{keys}}};")
         add(f"static __device__ __forceinline__ bool bg_prog_{self.gid}(const uint64_t *__restrict__ basis, "
             "uint64_t *__restrict__ result, uint32_t *status, int64_t W, uint32_t t, uint32_t epoch, "
-            "uint32_t n_carry, uint32_t *err, uint32_t *bgt, uint32_t phase) {")
+            "uint32_t n_carry, uint32_t *err, uint32_t *bgt, const uint32_t *__restrict__ bgf, uint32_t phase) {")
         add(f"  constexpr int HL = {self.HL}, HR = {self.HR}, S = {self.S};")
         add("  const int lane = bg_lane();")
         add("  const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;")
@@ -572,11 +603,20 @@ class GroupProgram:
                 add(f"  bas[{i}] = basis[(int64_t){sl} * W + (int64_t)wl];")
         add("#pragma unroll")
         add(f"  for (int i = 0; i < {len(slots)}; i++) bas[i] = inb ? bas[i] : 0ull;")
-        if nf:
+        if nf and shared:
+            # the answers of every filter test, from the set's filter pre-pass
+            # (bgf: this tile's words; wave-uniform)
+            add("  (void)bgt;")
+            add(f"  uint32_t bgm[{self.filter_words}];")
+            add("  bg_filter_words<" + str(self.filter_words) + ">(bgf, bgm);")
+        elif nf:
             # byte-pair table of the tile and the answers of every filter test
+            add("  (void)bgf;")
             add("  bg_bigram_build(bas, bgt);")
-            add(f"  uint32_t bgm[{2 * ((nf + 63) // 64)}];")
+            add(f"  uint32_t bgm[{self.filter_words}];")
             add(f"  bg_bigram_lookup<{nf}, {self.bigram_clauses}>(bgt, bg_keys_{self.gid}, bgm);")
+        else:
+            add("  (void)bgf;")
         # declarations (goto must not cross initialisations)
         vars_ = set()
         for inst in self.insts:
@@ -1001,17 +1041,20 @@ class GroupProgram:
             return f"__launch_bounds__({block_threads}, {self.min_waves})"
         return f"__launch_bounds__({block_threads})"
 
-    def table_decl(self, block_threads):
+    def table_decl(self, block_threads, shared=None):
         """The wave's byte-pair table in LDS (4 KiB per wave), or none."""
-        if not self.bigram_keys:
+        shared = self.shared_filter if shared is None else bool(shared)
+        if not self.bigram_keys or shared:
             return ["  uint32_t *bgt = nullptr;"]
         return [f"  __shared__ uint32_t bg_tables[{block_threads // WAVE}][BG_BIGRAM_WORDS];",
                 "  uint32_t *bgt = bg_tables[threadIdx.x / BG_WAVE];"]
 
     def source(self):
         """Plain form: one kernel per group, one launch per (group, buffer)."""
+        bgf = (f"a.filter + (size_t)t * {self.filter_row}u + {self.filter_off}u" if self.shared_filter
+               else "nullptr")
         call = (f"bg_ok = bg_prog_{self.gid}(a.basis, a.result, a.status, (int64_t)a.n_words, t, Q.epoch, "
-                "a.n_carry, a.ctl + BG_CTL_ERROR, bgt, phase);")
+                f"a.n_carry, a.ctl + BG_CTL_ERROR, bgt, {bgf}, phase);")
         chunk = [
             "const uint32_t tl = g + (uint32_t)bg_lane();",
             "uint32_t *row = tl < a.n_tiles ? a.status + (size_t)tl * a.n_carry : nullptr;",
@@ -1151,6 +1194,10 @@ class GroupProgram:
             "launches": 2 if self.two_launch else 1,
             "n_out": self.n_out,
             "n_bigram_tests": len(self.bigram_keys),
+            # byte-pair answers from the set's filter pre-pass: this kernel's
+            # word offset in a tile's row and the row length (0: none)
+            "filter_off": self.filter_off if self.shared_filter else 0,
+            "filter_row": self.filter_row if self.shared_filter else 0,
             "n_insts": len(self.insts),
             "regexes": list(self.regexes),
         }
@@ -1166,11 +1213,15 @@ def batch_source(program, block_threads=256):
     through ``tile_job``; each job's tiles are contiguous and increasing, so a
     tile's predecessor always has a smaller index)."""
     g = program.gid
+    # shared filters: the set's batched filter pre-pass wrote each global
+    # tile's answer row (the kernels of a set have the same tiles per job)
+    bgf = (f"a.filter + (size_t)t * {program.filter_row}u + {program.filter_off}u" if program.shared_filter
+           else "nullptr")
     run = [
         "const uint32_t j = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.tile_job[t]);",
         "const bg_job_t J = a.jobs[j];",
         f"bg_ok = bg_prog_{g}(J.basis, J.result, J.status, (int64_t)J.n_words, t - J.tile_start, Q.epoch, "
-        "J.n_carry, a.ctl + BG_CTL_ERROR, bgt, phase);",
+        f"J.n_carry, a.ctl + BG_CTL_ERROR, bgt, {bgf}, phase);",
     ]
     chunk = [
         "// lane i: global tile g + i, its job and row; lane 0's job for the",
@@ -1200,6 +1251,144 @@ def batch_source(program, block_threads=256):
         *program.two_launch_marker(batch_kernel_name(g)),
         "",
     ])
+
+
+FILTER_KERNEL = "bg_filter"
+FILTER_BLOCK_THREADS = 256
+
+
+def plan_shared_filter(programs):
+    """Give the kernels of a set that read shared byte-pair answers one tile
+    geometry (the widest halo of any of them) and their word offsets in a
+    tile's answer row.  Returns the row length (0 when no kernel shares)."""
+    progs = [p for p in programs if getattr(p, "shared_filter", False)]
+    if not progs:
+        return 0
+    HL = max(p.HL for p in progs)
+    HR = max(p.HR for p in progs)
+    off = 0
+    for p in progs:
+        p.set_geometry(HL, HR)
+        p.filter_off = off
+        off += p.filter_words
+    for p in progs:
+        p.filter_row = off
+    return off
+
+
+FILTER_BATCH_KERNEL = "bg_filter_batch"
+
+
+def filter_source(programs):
+    """The set's filter pre-pass, in two kernels: ``bg_filter`` (one buffer)
+    and ``bg_filter_batch`` (the global tiles of a batch, each mapped to its
+    job and the job's own tile index through ``tile_job``, as the batch
+    kernels map them).  Persistent waves take tiles (wave i of the grid:
+    tiles i, i + waves, ...); for each, load the tile's window of the basis
+    in the set's shared geometry, build its byte-pair table (the same
+    bg_bigram_build the kernels would run) and answer the filters of every
+    sharing kernel with that kernel's keys, then store the tile's answer
+    row: word filter_off + i of kernel k's answers is its bgm[i].  Lane f
+    holds the clauses of filter f (of each 64-filter chunk) in registers for
+    the whole launch.  The answers are those each kernel computes for itself
+    from the same window (bg_bigram_lookup's), so sharing changes no result."""
+    progs = [p for p in programs if getattr(p, "shared_filter", False)]
+    row = progs[0].filter_row
+    HL, HR, S = progs[0].HL, progs[0].HR, progs[0].S
+    nchunk = (row + WAVE - 1) // WAVE
+    wpb = FILTER_BLOCK_THREADS // WAVE
+    lines = ['#include "bitgen_device.h"']
+    for p in progs:
+        keys = ", ".join(f"0x{k:08X}u" for k in p.bigram_key_words())
+        lines.append(f"static __constant__ uint32_t bg_fkeys_{p.gid}[{len(p.bigram_key_words())}] = {{{keys}}};")
+
+    # the tile's answers (t: the tile within its buffer; basis, W: the buffer)
+    tile = [
+        "    const int64_t w0 = (t == 0) ? 0 : (int64_t)(64 - HR) + (int64_t)(t - 1) * S - HL;",
+        "    const int64_t w = w0 + lane;",
+        "    const bool inb = (w >= 0) && (w < W);",
+        "    const int64_t wc = w < 0 ? 0 : (w >= W ? W - 1 : w);",
+        "    uint64_t bas[8];",
+        "#pragma unroll",
+        "    for (int i = 0; i < 8; i++) bas[i] = basis[(int64_t)i * W + wc];",
+        "#pragma unroll",
+        "    for (int i = 0; i < 8; i++) bas[i] = inb ? bas[i] : 0ull;",
+        "    bg_bigram_build(bas, bgt);",
+        f"    uint32_t out[{nchunk}];",
+        "#pragma unroll",
+        f"    for (int c = 0; c < {nchunk}; c++) out[c] = 0u;",
+    ]
+    for p in progs:
+        nf, ncl = len(p.bigram_keys), p.bigram_clauses
+        for c in range((nf + WAVE - 1) // WAVE):
+            test = " && ".join([f"{c * 64} + lane < {nf}"]
+                               + [f"bg_bigram_clause(bgt, kr{p.gid}[{c}][{j}])" for j in range(ncl)])
+            # the fence keeps this chunk's table reads from being hoisted
+            # ahead of the earlier chunks' (each would hold registers)
+            tile += [
+                "    {",
+                "      " + " ".join(f"bg_fence(kr{p.gid}[{c}][{j}]);" for j in range(ncl)),
+                f"      const uint64_t m = __builtin_amdgcn_ballot_w64({test});",
+            ]
+            for half in range(2):
+                j = p.filter_off + 2 * c + half
+                val = "(uint32_t)m" if half == 0 else "(uint32_t)(m >> 32)"
+                tile.append(f"      out[{j // WAVE}] = lane == {j % WAVE} ? {val} : out[{j // WAVE}];")
+            tile.append("    }")
+    keys = ["  // every lane's clauses, once per wave"]
+    for p in progs:
+        nf, ncl = len(p.bigram_keys), p.bigram_clauses
+        nc = (nf + WAVE - 1) // WAVE
+        keys += [
+            f"  uint32_t kr{p.gid}[{nc}][{ncl}];",
+            "#pragma unroll",
+            f"  for (int c = 0; c < {nc}; c++)",
+            "#pragma unroll",
+            f"    for (int j = 0; j < {ncl}; j++)",
+            f"      kr{p.gid}[c][j] = c * 64 + lane < {nf} ? bg_fkeys_{p.gid}[(c * 64 + lane) * {ncl} + j] : 0u;",
+        ]
+    head = [
+        f"  __shared__ uint32_t bg_tables[{wpb}][BG_BIGRAM_WORDS];",
+        "  uint32_t *bgt = bg_tables[threadIdx.x / BG_WAVE];",
+        f"  constexpr int HL = {HL}, HR = {HR}, S = {S};",
+        "  const int lane = bg_lane();",
+        *keys,
+        f"  const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * {wpb}u + threadIdx.x / BG_WAVE));",
+        f"  const uint32_t stride = gridDim.x * {wpb}u;",
+    ]
+    store = [
+        "#pragma unroll",
+        f"    for (int c = 0; c < {nchunk}; c++)",
+        f"      if (c * 64 + lane < {row}) filter[(size_t)g * {row}u + c * 64 + lane] = out[c];",
+    ]
+    lines += [
+        f'extern "C" __global__ __launch_bounds__({FILTER_BLOCK_THREADS}) void {FILTER_KERNEL}('
+        "const uint64_t *__restrict__ basis, uint64_t n_words, uint32_t n_tiles, uint32_t *__restrict__ filter) {",
+        *head,
+        "  const int64_t W = (int64_t)n_words;",
+        "  for (uint32_t g = first; g < n_tiles; g += stride) {",
+        "    const uint32_t t = g;",
+        *tile,
+        *store,
+        "  }",
+        "}",
+        "",
+        f'extern "C" __global__ __launch_bounds__({FILTER_BLOCK_THREADS}) void {FILTER_BATCH_KERNEL}('
+        "const bg_job_t *__restrict__ jobs, const uint32_t *__restrict__ tile_job, uint32_t total_tiles, "
+        "uint32_t *__restrict__ filter) {",
+        *head,
+        "  for (uint32_t g = first; g < total_tiles; g += stride) {",
+        "    const bg_job_t J = jobs[(uint32_t)__builtin_amdgcn_readfirstlane((int)tile_job[g])];",
+        "    const uint64_t *__restrict__ basis = J.basis;",
+        "    const int64_t W = (int64_t)J.n_words;",
+        "    const uint32_t t = g - J.tile_start;",
+        *tile,
+        *store,
+        "  }",
+        "}",
+        "",
+    ]
+    return "\n".join(lines)
 
 
 def hsaco_cache_dir():

@@ -15,7 +15,7 @@ from typing import List
 
 from . import config as cfg
 from .bitparser import BitstreamParser
-from .codegen.gen_hip import GroupProgram, compile_sources
+from .codegen.gen_hip import FILTER_KERNEL, GroupProgram, compile_sources, filter_source, plan_shared_filter
 from .passes import (
     PassManager, star_rewrite_pass, cc_advance_pass, remove_alias_pass,
     short_circuit_pass, graph_break_pass,
@@ -60,6 +60,9 @@ DEFAULT_OPTIONS = {
     # per-tile statistics word (passes, tests passed) after the carry
     # statuses (the reference's --pass-inst-stats-dyn)
     "inst_stats_dyn": 0,
+    # byte-pair answers of all kernels of a set from one filter pre-pass
+    # (plain form; the kernels share one tile geometry)
+    "shared_filter": 0,
 }
 
 
@@ -217,7 +220,8 @@ def program_for(gid, insts, vnm, regexes, options):
                         min_waves=int(options["min_waves"]), bigram_filter=bool(options["bigram_filter"]),
                         replay_scope=options["replay_scope"], block_threads=int(options.get("block_threads", 256)),
                         carry_settle=options.get("carry_settle", "defer"),
-                        inst_stats_dyn=bool(options.get("inst_stats_dyn", 0)))
+                        inst_stats_dyn=bool(options.get("inst_stats_dyn", 0)),
+                        shared_filter=bool(options.get("shared_filter", 0)))
 
 
 @dataclass
@@ -296,12 +300,23 @@ def load_cached_set(d, name):
         m = dict(m, hsaco=os.path.abspath(os.path.join(hsaco_cache_dir(), m["hsaco"])))
         if not os.path.exists(m["hsaco"]):
             return None
+        if m.get("filter"):
+            m["filter"] = dict(m["filter"], hsaco=os.path.abspath(os.path.join(hsaco_cache_dir(),
+                                                                               m["filter"]["hsaco"])))
+            if not os.path.exists(m["filter"]["hsaco"]):
+                return None
         metas.append(m)
     return metas
 
 
 def store_cached_set(d, name, metas):
-    rec = {"digest": code_digest(), "metas": [dict(m, hsaco=os.path.basename(m["hsaco"])) for m in metas]}
+    def rel(m):
+        m = dict(m, hsaco=os.path.basename(m["hsaco"]))
+        if m.get("filter"):
+            m["filter"] = dict(m["filter"], hsaco=os.path.basename(m["filter"]["hsaco"]))
+        return m
+
+    rec = {"digest": code_digest(), "metas": [rel(m) for m in metas]}
     path = _index_path(d, name)
     with open(path + ".tmp", "w") as f:
         json.dump(rec, f)
@@ -334,7 +349,7 @@ def kernel_programs(groups, o):
 
                 prog = StreamGroup(g, insts, vnm, groups[g], make)
             else:
-                prog = program_for(g, insts, vnm, groups[g], o)
+                prog = program_for(g, insts, vnm, groups[g], dict(o, shared_filter=0))
             prog.groups = [g]
             programs.append(prog)
         return programs
@@ -347,6 +362,7 @@ def kernel_programs(groups, o):
         prog = program_for(kid, insts, vnm, [r for g in members for r in groups[g]], o)
         prog.groups = members
         programs.append(prog)
+    plan_shared_filter(programs)
     return programs
 
 
@@ -375,11 +391,21 @@ def compile_pattern_set(groups, options=None, workers=None):
         with open(path, "w") as f:
             f.write(src)
         sources[path] = src
+    row = max((getattr(p, "filter_row", 0) for p in programs if getattr(p, "shared_filter", False)), default=0)
+    fpath = os.path.join(d, "filter.hip")
+    if row:
+        src = filter_source(programs)
+        with open(fpath, "w") as f:
+            f.write(src)
+        sources[fpath] = src
     hsacos = compile_sources(sources, workers=workers)
     metas = []
     for prog in programs:
         m = kernel_meta(prog)
         m["hsaco"] = os.path.abspath(hsacos[os.path.join(d, f"kernel_{prog.gid}.hip")])
+        if row:
+            # the set's filter pre-pass (one code object for the set)
+            m["filter"] = {"hsaco": os.path.abspath(hsacos[fpath]), "kernel": FILTER_KERNEL, "row": row}
         metas.append(m)
     store_cached_set(d, "plain.json", metas)
     return CompiledPatternSet(groups, metas, d, o)

@@ -406,10 +406,49 @@ int bg_group_launch_static(bg_kernel_t k, const uint64_t *basis, uint64_t *resul
   a.n_queues = 0;
   a.n_carry = n_carry;
   a.take = 0;
+  a.filter = nullptr;
   int rc = bg_group_launch(k, &a, 0, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(bg_take_error_kernel, dim3(1), dim3(64), 0, S(stream), s + BG_CTL_ERROR, error_flag);
   BG_HIP(hipGetLastError(), "bg_group_launch_static error hand-off");
+  return 0;
+}
+
+int bg_filter_launch(bg_kernel_t k, const uint64_t *basis, uint64_t n_words, uint32_t n_tiles, uint32_t *filter,
+                     uint32_t grid_blocks, void *stream) {
+  if (!k || !basis || !filter) return set_err(BG_EINVAL, "bg_filter_launch: null argument");
+  if (n_tiles == 0) return 0;
+  if (k->threads == 0) {
+    int v = 0;
+    BG_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn), "max threads");
+    k->threads = (v >= 64 && v <= 1024 && v % 64 == 0) ? (uint32_t)v : 256u;
+  }
+  // persistent: the blocks that fit at once (never more than one wave per
+  // tile); wave i of the grid takes tiles i, i + waves, ...
+  uint32_t blocks = 0;
+  int rc = persistent_grid(k, k->threads, n_tiles, 1, grid_blocks, &blocks);
+  if (rc) return rc;
+  void *params[] = {&basis, &n_words, &n_tiles, &filter};
+  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, k->threads, 1, 1, 0, S(stream), params, nullptr),
+         "hipModuleLaunchKernel(filter)");
+  return 0;
+}
+
+int bg_filter_batch_launch(bg_kernel_t k, const bg_job_t *jobs, const uint32_t *tile_job, uint32_t total_tiles,
+                           uint32_t *filter, uint32_t grid_blocks, void *stream) {
+  if (!k || !jobs || !tile_job || !filter) return set_err(BG_EINVAL, "bg_filter_batch_launch: null argument");
+  if (total_tiles == 0) return 0;
+  if (k->threads == 0) {
+    int v = 0;
+    BG_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, k->fn), "max threads");
+    k->threads = (v >= 64 && v <= 1024 && v % 64 == 0) ? (uint32_t)v : 256u;
+  }
+  uint32_t blocks = 0;
+  int rc = persistent_grid(k, k->threads, total_tiles, 1, grid_blocks, &blocks);
+  if (rc) return rc;
+  void *params[] = {&jobs, &tile_job, &total_tiles, &filter};
+  BG_HIP(hipModuleLaunchKernel(k->fn, blocks, 1, 1, k->threads, 1, 1, 0, S(stream), params, nullptr),
+         "hipModuleLaunchKernel(filter batch)");
   return 0;
 }
 

@@ -180,6 +180,11 @@ class MatchServer:
         for sid, ps in self.sets.items():
             if ps.kernels is None:
                 raise ValueError(f"pattern set {sid} is not loaded (call BatchPatternSet.load first)")
+            if ps.filter_row:
+                # the native staging launches the set's kernels only; their
+                # byte-pair tables must be built in the kernels
+                raise ValueError(f"pattern set {sid} shares its byte-pair filters across kernels (shared_filter); "
+                                 "served sets are compiled with shared_filter=0")
             for kid, kern in enumerate(ps.kernels):
                 occ[(sid, kid)] = rt.kernel_occupancy(kern, ps.block_threads)
         self.grid_blocks_max, self.exclusive_sets = grid_caps(occ, n_streams)

@@ -120,6 +120,7 @@ __device__ __forceinline__ uint64_t bg_advl(uint64_t x) {
 // before this point with values computed after it: issued at regex
 // boundaries so every regex recomputes its classes next to their use.
 __device__ __forceinline__ void bg_fence(uint64_t &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void bg_fence(uint32_t &x) { asm volatile("" : "+v"(x)); }
 
 // Scalar counterpart, for wave-uniform values: issued at the top of every
 // pass so that tests derived from x are not all hoisted in front of the
@@ -209,6 +210,27 @@ __device__ __forceinline__ void bg_bigram_lookup(const uint32_t *tab, const uint
     masks[2 * c] = (uint32_t)m;
     masks[2 * c + 1] = (uint32_t)(m >> 32);
   }
+}
+
+// A tile's byte-pair answers from the pattern set's filter pre-pass: n words
+// at a wave-uniform address, written by an earlier launch, read through the
+// constant address space so that they arrive as scalar loads.
+template <int N>
+__device__ __forceinline__ void bg_filter_words(const uint32_t *src, uint32_t *masks) {
+  const __attribute__((address_space(4))) uint32_t *p = (const __attribute__((address_space(4))) uint32_t *)src;
+#pragma unroll
+  for (int i = 0; i < N; i++) masks[i] = p[i];
+}
+
+// One clause of a filter against the tile's table: up to two 15-bit pair
+// hashes (low and high half, 0xFFFF = none), one of which must be present;
+// 0xFFFFFFFF = an unused clause (always true).
+__device__ __forceinline__ bool bg_bigram_clause(const uint32_t *tab, uint32_t k) {
+  if (k == 0xFFFFFFFFu) return true;
+  const uint32_t a = k & 0xFFFFu, b = k >> 16;
+  bool h = (tab[a >> 5] >> (a & 31u)) & 1u;
+  if (b != 0xFFFFu) h = h || ((tab[b >> 5] >> (b & 31u)) & 1u);
+  return h;
 }
 
 // ---- three-input bitwise functions ---------------------------------------------

@@ -66,6 +66,10 @@ typedef struct bg_group_args {
                             blocks)                                          */
   uint32_t n_carry;      /* carry operations in the program (>= 1 slot)      */
   uint32_t take;         /* tiles per atomic take, 0 = default; 1 .. 8       */
+  const uint32_t *filter;/* [n_tiles][row] byte-pair answers of the pattern
+                            set's filter pre-pass (bg_filter_launch), read by
+                            kernels compiled with shared filters; else unused
+                            (may be null)                                    */
 } bg_group_args_t;
 
 /* One (kernel, buffer) job of a batched launch.  Field order is ABI. */
@@ -93,6 +97,10 @@ typedef struct bg_batch_args {
   uint32_t total_tiles;
   uint32_t n_queues;        /* 0 = default; clamped like bg_group_args     */
   uint32_t take;            /* tiles per atomic take, as in bg_group_args  */
+  const uint32_t *filter;   /* [total_tiles][row] byte-pair answers of the
+                               set's batched filter pre-pass
+                               (bg_filter_batch_launch), read by kernels
+                               compiled with shared filters; else unused    */
 } bg_batch_args_t;
 
 typedef struct bg_graph_s *bg_graph_t;
@@ -170,6 +178,27 @@ uint32_t bg_ctl_words(void);
  * asynchronously: the caller synchronises. */
 int bg_group_launch(bg_kernel_t kernel, const bg_group_args_t *args, uint32_t grid_blocks,
                     void *stream);
+
+/* Filter pre-pass of a pattern set whose kernels share their byte-pair
+ * answers (no counterpart in the reference: the byte-pair filters are this
+ * backend's own).  One wave per tile of the set's shared tile geometry
+ * builds the tile's pair table once and writes the answers of every
+ * kernel's filters to filter[n_tiles][row] (row: the set's answer words per
+ * tile); the set's group kernels then read their words instead of each
+ * building the table.  Launch it on the same stream before the group
+ * kernels of the same buffer. */
+int bg_filter_launch(bg_kernel_t kernel, const uint64_t *basis, uint64_t n_words, uint32_t n_tiles,
+                     uint32_t *filter, uint32_t grid_blocks, void *stream);
+
+/* The same pre-pass over the global tiles of a batch (kernel
+ * bg_filter_batch): tile g belongs to job tile_job[g] of the jobs table of
+ * any kernel of the set (the kernels of a set that share filters have the
+ * same tiles per job), and its answer row lands at filter[g][row].  A
+ * kernel node only, so it can be captured in a graph.  grid_blocks (both
+ * entries): 0 = the blocks that fit on the device at once, each wave taking
+ * every (waves)-th tile; else that many blocks (at most one wave per tile). */
+int bg_filter_batch_launch(bg_kernel_t kernel, const bg_job_t *jobs, const uint32_t *tile_job,
+                           uint32_t total_tiles, uint32_t *filter, uint32_t grid_blocks, void *stream);
 
 /* LSB-first u64 words -> the reference's u32 layout (position p = bit
  * 31 - p%32 of word p/32, bitgen/bitstream.py:30-52).  dst holds 2*n_words

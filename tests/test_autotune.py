@@ -16,8 +16,10 @@ def test_candidates_are_distinct_and_clamped():
 def test_options_of_candidate_keep_the_base():
     o = autotune.options_of((8, 5, 64, 1, 512), {"lookback": "speculate", "min_waves": 2})
     assert o == {"lookback": "speculate", "groups_per_kernel": 8, "min_waves": 5, "bigram_filter": 1,
-                 "block_threads": 512}
-    for key in ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads"):
+                 "block_threads": 512, "shared_filter": 0}
+    assert autotune.options_of((4, 5, 64, 2, 256))["shared_filter"] == 1
+    assert autotune.options_of((4, 5, 64, 2, 256))["bigram_filter"] == 1
+    for key in ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads", "shared_filter"):
         assert key in compiler.DEFAULT_OPTIONS
 
 

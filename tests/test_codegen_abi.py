@@ -582,3 +582,63 @@ def test_tile_model_counts_tests_that_fall_through():
     assert len(stats) == p.n_tiles(O.n_words_for(data.size))
     # every test of the literal passes in the tile holding it, none elsewhere
     assert sorted(stats) == [0] * (len(stats) - 1) + [n_tests] and n_tests > 0
+
+
+def test_shared_filter_geometry_and_sources():
+    """shared_filter: the kernels of a set take the widest halo of any of
+    them (carry entry lanes move with HL), their answer words sit one after
+    another in a tile's row, each kernel reads its words (no table build in
+    the kernel) and the set's filter pre-pass builds the table once and
+    answers every kernel's filters with that kernel's own keys."""
+    from bitgen_amd.codegen.gen_hip import filter_source
+
+    groups = [["ab[0-9]+c", "hello"], ["x.{0,40}y", "[ab]cd"], ["zz[^\\n]*yy", "q0q0"]]
+    o = dict(compiler.DEFAULT_OPTIONS, groups_per_kernel=1, bigram_filter=1)
+    own = compiler.kernel_programs(groups, o)
+    shared = compiler.kernel_programs(groups, dict(o, shared_filter=1))
+    HL = max(p.HL for p in own)
+    HR = max(p.HR for p in own)
+    assert {(p.HL, p.HR, p.S) for p in shared} == {(HL, HR, 64 - HL - HR)}
+    off = 0
+    for a, b in zip(own, shared):
+        assert b.shared_filter and b.filter_off == off and b.filter_words == a.filter_words
+        off += b.filter_words
+        assert [L - a.HL for _, L in a.carry_ops] == [L - b.HL for _, L in b.carry_ops]
+        src = b.source()
+        assert "bg_filter_words<" in src and "bg_bigram_build" not in src and "a.filter +" in src
+        assert "bg_bigram_build" in a.source() and "bg_filter_words<" not in a.source()
+    assert all(p.filter_row == off for p in shared)
+    fsrc = filter_source(shared)
+    # one table build per tile in each of the two pre-pass kernels (one
+    # buffer, a batch's global tiles)
+    assert fsrc.count("bg_bigram_build(") == 2
+    assert 'void bg_filter(' in fsrc and 'void bg_filter_batch(' in fsrc
+    for p in shared:
+        # lane f holds filter f's clauses; one ballot per 64 filters, two
+        # answer words per ballot at the kernel's offset
+        assert f"kr{p.gid}[c][j] = c * 64 + lane < {len(p.bigram_keys)} ? bg_fkeys_{p.gid}[" in fsrc
+        nchunks = (len(p.bigram_keys) + 63) // 64
+        for c in range(nchunks):
+            assert f"bg_bigram_clause(bgt, kr{p.gid}[{c}][0])" in fsrc
+        for j in range(p.filter_off, p.filter_off + p.filter_words):
+            assert f"out[{j // 64}] = lane == {j % 64} ?" in fsrc
+    # the batch kernels read the batched pre-pass's rows by global tile
+    from bitgen_amd.codegen.gen_hip import batch_source
+
+    bsrc = batch_source(shared[1])
+    assert "bg_bigram_build" not in bsrc and "bg_filter_words<" in bsrc
+    assert f"a.filter + (size_t)t * {shared[1].filter_row}u + {shared[1].filter_off}u" in bsrc
+    assert "bg_bigram_build" in batch_source(own[1])
+
+
+def test_shared_filter_set_compiles_with_filter_code_object():
+    cs = compiler.compile_pattern_set([["hello", "ab[0-9]+c"], ["x[yz]q"]],
+                                      {"bigram_filter": 1, "shared_filter": 1, "groups_per_kernel": 1})
+    rows = {m["filter_row"] for m in cs.metas}
+    assert len(rows) == 1 and rows.pop() > 0
+    f = cs.metas[0]["filter"]
+    assert f["kernel"] == "bg_filter" and os.path.exists(f["hsaco"])
+    # cached: the filter code object comes back from the index
+    again = compiler.compile_pattern_set([["hello", "ab[0-9]+c"], ["x[yz]q"]],
+                                         {"bigram_filter": 1, "shared_filter": 1, "groups_per_kernel": 1})
+    assert again.metas[0]["filter"]["hsaco"] == f["hsaco"]

@@ -106,6 +106,35 @@ def test_batched_bigram_filter(gpu):
 
 
 
+def test_batched_shared_filter_eager_and_graph(gpu):
+    """Sets whose kernels share their byte-pair answers: one batched filter
+    pre-pass per set over the batch's global tiles (each mapped to its job's
+    buffer and tile), then the set's kernels read their rows; eager and
+    graph replays (the pre-pass is a kernel node), every job bit-exact."""
+    import torch
+
+    sets = [BatchPatternSet(s, {"bigram_filter": 1, "shared_filter": 1, "groups_per_kernel": k}).load(0)
+            for s, k in zip(SETS, (1, 2))]
+    assert all(s.filter_row > 0 and s.filter_kernel is not None for s in sets)
+    buffers = _buffers(9, 14)
+    # a partial job list: not every buffer in every set
+    jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers)) if (si + b) % 3]
+    print(f"buffers={[b.size for b in buffers]}", flush=True)
+    plan = BatchPlan(sets, buffers, jobs)
+    plan.enqueue()
+    _check(plan, sets, buffers, jobs)
+    plan.capture()
+    for rep in range(3):
+        plan.result.fill_(0)
+        plan.run_graph()
+        torch.cuda.synchronize()
+        _check(plan, sets, buffers, jobs)
+        _ = (plan.bytes[:128].to(torch.int32) * 3).sum().item()  # unrelated eager work
+    plan.destroy()
+    for s in sets:
+        s.free()
+
+
 @pytest.mark.parametrize("settle", ["launch", "defer"])
 def test_batched_phased_graph(gpu, settle):
     """Batch kernels in the sweep + settle form (both launches of every

@@ -69,13 +69,15 @@ BIGRAM_POOL = ["hello", "ab[0-9]+c", "x[yz]q", "q0q0", "[ab]cd", "zz[^\\n]*yy", 
                "dcba(xy|zq)", "c?abc", "^abc", "xyz$", "bb[0-9]{2,4}c", "\\x81\\x02", "[0-9]{2,4}"]
 
 
-@pytest.mark.parametrize("seed,per_kernel,min_waves",
-                         [(s, k, 0) for s in range(8) for k in (1, 2)] + [(s, 4, 5) for s in range(8)])
-def test_bigram_filter_groups(gpu, seed, per_kernel, min_waves):
+@pytest.mark.parametrize("seed,per_kernel,min_waves,shared",
+                         [(s, k, 0, 0) for s in range(8) for k in (1, 2)] + [(s, 4, 5, 0) for s in range(8)]
+                         + [(s, k, 5, 1) for s in range(8) for k in (1, 4)])
+def test_bigram_filter_groups(gpu, seed, per_kernel, min_waves, shared):
     """Per-tile byte-pair filters in front of literal first stages: tiles
     that hold a stage's pair and tiles that do not (printable text with
     planted literals, bytes above 127 whose pairs share a hash with ASCII
-    pairs), single and merged kernels, every group bit-exact."""
+    pairs), single and merged kernels, the pair tables built in each kernel
+    or once per set by the filter pre-pass (shared), every group bit-exact."""
     rng = random.Random(200 + seed)
     g = np.random.default_rng(200 + seed)
     n = rng.choice([60, 5000, 70000, 300001])
@@ -87,9 +89,11 @@ def test_bigram_filter_groups(gpu, seed, per_kernel, min_waves):
         off = rng.randrange(0, max(1, n - lit.size))
         data[off:off + lit.size] = lit[:n - off]
     groups = [rng.sample(BIGRAM_POOL, rng.randint(1, 5)) for _ in range(rng.randint(1, 4) + (4 if per_kernel == 4 else 0))]
-    print(f"seed={seed} per_kernel={per_kernel} min_waves={min_waves} n={n} groups={groups}", flush=True)
+    print(f"seed={seed} per_kernel={per_kernel} min_waves={min_waves} shared={shared} n={n} groups={groups}",
+          flush=True)
     assert_parity(groups, data, gpu, options={"bigram_filter": 1, "groups_per_kernel": per_kernel,
-                                              "min_waves": min_waves}, repeat=2, label=f"bigram seed {seed}")
+                                              "min_waves": min_waves, "shared_filter": shared},
+                  repeat=2, label=f"bigram seed {seed}")
 
 
 @pytest.mark.parametrize("per_kernel", [1, 2])

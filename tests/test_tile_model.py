@@ -117,7 +117,8 @@ def test_random_groups_all_shapes(seed):
     data = np.random.default_rng(700 + seed).choice(alpha, size=n, p=p)
     pool = POOL + BIGRAM_POOL
     groups = [rng.sample(pool, rng.randint(1, 5)) for _ in range(rng.randint(1, 5))]
-    for opts in ({}, {"groups_per_kernel": 2}, {"groups_per_kernel": 4, "bigram_filter": 1}):
+    for opts in ({}, {"groups_per_kernel": 2}, {"groups_per_kernel": 4, "bigram_filter": 1},
+                 {"groups_per_kernel": 2, "bigram_filter": 1, "shared_filter": 1}):
         _assert_model(groups, data, opts, label=f"seed {seed} {opts}")
 
 
