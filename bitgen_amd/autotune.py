@@ -56,7 +56,7 @@ CANDIDATES = [
 
 # the shape chosen for the IDS set on MI355X so far; compiled ahead of time
 # for the sets that reuse the IDS choice
-USUAL = (4, 5, 64, 1, 256)
+USUAL = (4, 5, 64, 2, 256)
 
 # candidates timed within this fraction of the fastest are ties (the spread
 # of repeated timings on MI355X is 1-3%); a tie goes to the candidate listed

@@ -1071,6 +1071,7 @@ class GroupProgram:
             *self.work_loop("a.n_tiles", [call], chunk),
             "}",
             *self.two_launch_marker(self.kernel_name),
+            *self.shared_filter_marker(self.kernel_name),
             "",
         ])
 
@@ -1178,6 +1179,14 @@ class GroupProgram:
             return []
         return [f'extern "C" __global__ void {kernel}_two_launch() {{}}']
 
+    def shared_filter_marker(self, kernel):
+        """An empty kernel named <kernel>_shared_filter: its presence tells
+        the runtime that `kernel` reads its byte-pair answers from a filter
+        pre-pass (a launch without them is refused)."""
+        if not self.shared_filter:
+            return []
+        return [f'extern "C" __global__ void {kernel}_shared_filter() {{}}']
+
     def meta(self):
         return {
             "gid": self.gid,
@@ -1249,6 +1258,7 @@ def batch_source(program, block_threads=256):
         *program.work_loop("a.total_tiles", run, chunk),
         "}",
         *program.two_launch_marker(batch_kernel_name(g)),
+        *program.shared_filter_marker(batch_kernel_name(g)),
         "",
     ])
 

@@ -50,6 +50,7 @@ struct bg_kernel_s {
   int32_t occ_blocks;    // blocks per CU at that block size
   uint32_t threads;      // block size of a plain launch: the kernel's launch bounds (0 = not read yet)
   uint32_t launches;     // 2: a sweep and a settle launch per run (phase argument 0, then 1)
+  uint32_t needs_filter; // 1: reads its byte-pair answers from a filter pre-pass (args filter)
 };
 
 struct bg_graph_s {
@@ -275,7 +276,15 @@ int bg_kernel_get(bg_module_t module, const char *name, bg_kernel_t *out) {
     if (hipModuleGetFunction(&m, module->mod, marker) == hipSuccess) launches = 2;
     (void)hipGetLastError();  // a missing marker is not an error
   }
-  *out = new bg_kernel_s{f, 0, 0, 0, launches};
+  // a kernel compiled with shared byte-pair filters comes with an empty
+  // marker kernel <name>_shared_filter: it must be given its answers
+  uint32_t needs_filter = 0;
+  if (snprintf(marker, sizeof(marker), "%s_shared_filter", name) < (int)sizeof(marker)) {
+    hipFunction_t m;
+    if (hipModuleGetFunction(&m, module->mod, marker) == hipSuccess) needs_filter = 1;
+    (void)hipGetLastError();
+  }
+  *out = new bg_kernel_s{f, 0, 0, 0, launches, needs_filter};
   return 0;
 }
 
@@ -359,6 +368,9 @@ int bg_group_launch(bg_kernel_t k, const bg_group_args_t *args, uint32_t grid_bl
   if (!k || !args) return set_err(BG_EINVAL, "bg_group_launch: null argument");
   if (!args->basis || !args->result || !args->status || !args->ctl)
     return set_err(BG_EINVAL, "bg_group_launch: null buffer");
+  if (k->needs_filter && !args->filter)
+    return set_err(BG_EINVAL, "bg_group_launch: the kernel reads shared byte-pair answers; pass the buffer "
+                              "bg_filter_launch filled as args->filter");
   if (args->n_tiles == 0) return 0;
   if (k->threads == 0) {
     // the block size the kernel was compiled for (its launch bounds): one
@@ -389,6 +401,9 @@ int bg_group_launch_static(bg_kernel_t k, const uint64_t *basis, uint64_t *resul
   uint32_t *s = (dev >= 0 && dev < BG_MAX_DEVICES) ? g_scratch[dev] : nullptr;
   if (!s) return set_err(BG_EINVAL, "bg_group_launch_static: call bg_init for this device first");
   if (!k || !basis || !result || !error_flag) return set_err(BG_EINVAL, "bg_group_launch_static: null argument");
+  if (k->needs_filter)
+    return set_err(BG_EINVAL, "bg_group_launch_static: kernels compiled with shared byte-pair filters need "
+                              "their filter pre-pass; use bg_group_launch");
   if (n_carry == 0) return set_err(BG_EINVAL, "bg_group_launch_static: n_carry must be at least 1");
   if ((uint64_t)n_tiles * n_carry > BG_STATIC_STATUS_WORDS) {
     snprintf(g_err, sizeof(g_err),
@@ -479,6 +494,9 @@ int bg_batch_launch(bg_kernel_t k, const bg_batch_args_t *args, uint32_t grid_bl
                     void *stream) {
   if (!k || !args || !args->jobs || !args->tile_job || !args->ctl)
     return set_err(BG_EINVAL, "bg_batch_launch: null argument");
+  if (k->needs_filter && !args->filter)
+    return set_err(BG_EINVAL, "bg_batch_launch: the kernel reads shared byte-pair answers; pass the buffer "
+                              "bg_filter_batch_launch filled as args->filter");
   if (block_threads == 0 || block_threads % 64 || block_threads > 1024)
     return set_err(BG_EINVAL, "bg_batch_launch: block_threads must be a multiple of 64, at most 1024");
   if (args->total_tiles == 0) return 0;

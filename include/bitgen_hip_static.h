@@ -30,7 +30,9 @@ extern "C" {
  * look-back error word into *error_flag (device u32, 0 = results valid) and
  * clears it.  Every call on a device shares that device's scratch: calls
  * that may overlap on one device (two streams, two threads) race and must be
- * ordered by the caller, e.g. on one stream. */
+ * ordered by the caller, e.g. on one stream.  Kernels compiled with shared
+ * byte-pair filters (they need their set's filter pre-pass and its buffer)
+ * are refused with BG_EINVAL. */
 int bg_group_launch_static(bg_kernel_t kernel, const uint64_t *basis, uint64_t *result, uint64_t n_words,
                            uint32_t n_tiles, uint32_t n_carry, uint32_t *error_flag, void *stream);
 

@@ -606,6 +606,8 @@ def test_shared_filter_geometry_and_sources():
         assert [L - a.HL for _, L in a.carry_ops] == [L - b.HL for _, L in b.carry_ops]
         src = b.source()
         assert "bg_filter_words<" in src and "bg_bigram_build" not in src and "a.filter +" in src
+        # the marker that makes the runtime refuse a launch without answers
+        assert f"void {b.kernel_name}_shared_filter()" in src and "_shared_filter()" not in a.source()
         assert "bg_bigram_build" in a.source() and "bg_filter_words<" not in a.source()
     assert all(p.filter_row == off for p in shared)
     fsrc = filter_source(shared)

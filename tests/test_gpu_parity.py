@@ -242,3 +242,34 @@ def test_phased_settle(gpu, seed, settle):
             "bigram_filter": seed % 2 if pool is FUSED_POOL else 0, "min_waves": 5 if seed % 4 == 3 else 0}
     print(f"seed={seed} n={n} opts={opts} groups={groups}", flush=True)
     assert_parity(groups, data, gpu, options=opts, repeat=2, label=f"{settle} seed {seed}")
+
+
+def test_shared_filter_kernels_refuse_launch_without_answers(gpu):
+    """A kernel compiled with shared byte-pair filters is refused (an error,
+    before anything is launched) by bg_group_launch without the answer
+    buffer and by bg_group_launch_static; the set's own run (pre-pass, then
+    kernels) is bit-exact."""
+    import torch
+
+    from bitgen_amd import compiler
+    from bitgen_amd.backend.hip import runtime as rt
+
+    groups = [["hello", "ab[0-9]+c"], ["x[yz]q", "zz[^\\n]*yy"]]
+    data = np.frombuffer(b"hello ab12c xyq zzkkyy\n" * 300, dtype=np.uint8)
+    assert_parity(groups, data, gpu, options={"bigram_filter": 1, "shared_filter": 1}, repeat=2, label="shared")
+    ks = compiler.compile_pattern_set(groups, {"bigram_filter": 1, "shared_filter": 1}).load(0)
+    buf = torch.zeros(8 * 64, dtype=torch.uint64, device=gpu)
+    status = ks.alloc_status(1, 64)
+    args = rt.GroupArgs()
+    args.basis = args.result = buf.data_ptr()
+    args.status = status.data_ptr()
+    args.ctl = ks.ctl[0].data_ptr()
+    args.n_words, args.n_tiles, args.n_carry = 64, 1, ks.metas[0]["status_slots"]
+    with pytest.raises(rt.HipRuntimeError, match="shared byte-pair answers"):
+        rt.group_launch(ks.kernels[0], args, rt.current_stream_ptr())
+    flag = torch.zeros(1, dtype=torch.int32, device=gpu)
+    with pytest.raises(rt.HipRuntimeError, match="shared byte-pair filters"):
+        rt.group_launch_static(ks.kernels[0], buf.data_ptr(), buf.data_ptr(), 64, 1, 1, flag.data_ptr(),
+                               rt.current_stream_ptr())
+    torch.cuda.synchronize()
+    ks.free()
