@@ -1270,9 +1270,13 @@ FILTER_BLOCK_THREADS = 256
 def plan_shared_filter(programs):
     """Give the kernels of a set that read shared byte-pair answers one tile
     geometry (the widest halo of any of them) and their word offsets in a
-    tile's answer row.  Returns the row length (0 when no kernel shares)."""
+    tile's answer row.  Returns the row length (0 when no kernel shares).
+    A set with a single kernel that has filters shares nothing (the
+    pre-pass would only add a launch): that kernel builds its own table."""
     progs = [p for p in programs if getattr(p, "shared_filter", False)]
-    if not progs:
+    if len(progs) < 2:
+        for p in progs:
+            p.shared_filter = False
         return 0
     HL = max(p.HL for p in progs)
     HR = max(p.HR for p in progs)
