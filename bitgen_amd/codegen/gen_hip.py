@@ -420,17 +420,17 @@ class GroupProgram:
         self.insts = out
 
     def bigram_key_words(self):
-        """Filter f's clause c as one word (two 15-bit pair hashes, low and
-        high half, 0xFFFF = none; 0xFFFFFFFF = an unused clause that always
-        holds), bigram_clauses words per filter."""
+        """Filter f's clause c as one word: two 15-bit pair hashes (low and
+        high half), one of which must be in the tile's table;
+        bigram_clauses words per filter.  A clause of one pair repeats it in
+        both halves (a or a = a), and a filter with fewer clauses repeats
+        its first clause (the clauses are ANDed), so every word is tested
+        the same way, with no branch (bg_bigram_clause)."""
         words = []
         for clauses in self.bigram_keys:
             for c in range(self.bigram_clauses):
-                if c < len(clauses):
-                    h = clauses[c]
-                    words.append(h[0] | ((h[1] if len(h) > 1 else 0xFFFF) << 16))
-                else:
-                    words.append(0xFFFFFFFF)
+                h = clauses[c] if c < len(clauses) else clauses[0]
+                words.append(h[0] | ((h[1] if len(h) > 1 else h[0]) << 16))
         return words
 
     def _const(self, v):
@@ -1335,14 +1335,15 @@ def filter_source(programs):
     for p in progs:
         nf, ncl = len(p.bigram_keys), p.bigram_clauses
         for c in range((nf + WAVE - 1) // WAVE):
-            test = " && ".join([f"{c * 64} + lane < {nf}"]
-                               + [f"bg_bigram_clause(bgt, kr{p.gid}[{c}][{j}])" for j in range(ncl)])
+            # every clause evaluated (no short-circuit: no branch per clause)
+            test = " & ".join([f"(uint32_t)({c * 64} + lane < {nf})"]
+                              + [f"bg_bigram_clause(bgt, kr{p.gid}[{c}][{j}])" for j in range(ncl)])
             # the fence keeps this chunk's table reads from being hoisted
             # ahead of the earlier chunks' (each would hold registers)
             tile += [
                 "    {",
                 "      " + " ".join(f"bg_fence(kr{p.gid}[{c}][{j}]);" for j in range(ncl)),
-                f"      const uint64_t m = __builtin_amdgcn_ballot_w64({test});",
+                f"      const uint64_t m = __builtin_amdgcn_ballot_w64(({test}) != 0u);",
             ]
             for half in range(2):
                 j = p.filter_off + 2 * c + half

@@ -182,31 +182,31 @@ __device__ __forceinline__ void bg_bigram_build(const uint64_t bas[8], uint32_t 
   }
 }
 
+// One clause of a filter against the tile's table: two 15-bit pair hashes
+// (low and high half; a one-pair clause repeats its pair, a filter with
+// fewer clauses repeats its first), one of which must be present.  No
+// branch: both table words are read.
+__device__ __forceinline__ uint32_t bg_bigram_clause(const uint32_t *tab, uint32_t k) {
+  const uint32_t a = k & 0xFFFFu, b = k >> 16;
+  return ((tab[a >> 5] >> (a & 31u)) | (tab[b >> 5] >> (b & 31u))) & 1u;
+}
+
 // Wave-uniform answers of NF filters of NCL clauses each: bit f % 32 of
 // masks[f / 32] is set when every clause of filter f may hold in the tile.
-// keys[f * NCL + c] holds clause c: up to two 15-bit pair hashes (low and
-// high half, 0xFFFF = none), one of which must be in the table;
-// 0xFFFFFFFF = an unused clause.  32-bit masks keep each test a single
-// scalar AND (its result sets SCC for the branch).
+// keys[f * NCL + c] holds clause c (bg_bigram_clause: two 15-bit pair
+// hashes, one of which must be in the table).  32-bit masks keep each test a
+// single scalar AND (its result sets SCC for the branch).
 template <int NF, int NCL>
 __device__ __forceinline__ void bg_bigram_lookup(const uint32_t *tab, const uint32_t *keys, uint32_t *masks) {
   const int lane = bg_lane();
 #pragma unroll
   for (int c = 0; c < (NF + 63) / 64; c++) {
     const int f = c * 64 + lane;
-    bool hit = f < NF;
-    if (hit) {
+    const int fc = f < NF ? f : NF - 1;  // lanes past the last filter: any key, answer dropped
+    uint32_t hit = f < NF;
 #pragma unroll
-      for (int j = 0; j < NCL; j++) {
-        const uint32_t k = keys[f * NCL + j];
-        if (k == 0xFFFFFFFFu) continue;
-        const uint32_t a = k & 0xFFFFu, b = k >> 16;
-        bool h = (tab[a >> 5] >> (a & 31u)) & 1u;
-        if (b != 0xFFFFu) h = h || ((tab[b >> 5] >> (b & 31u)) & 1u);
-        hit = hit && h;
-      }
-    }
-    const uint64_t m = __builtin_amdgcn_ballot_w64(hit);
+    for (int j = 0; j < NCL; j++) hit &= bg_bigram_clause(tab, keys[fc * NCL + j]);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(hit != 0u);
     masks[2 * c] = (uint32_t)m;
     masks[2 * c + 1] = (uint32_t)(m >> 32);
   }
@@ -220,17 +220,6 @@ __device__ __forceinline__ void bg_filter_words(const uint32_t *src, uint32_t *m
   const __attribute__((address_space(4))) uint32_t *p = (const __attribute__((address_space(4))) uint32_t *)src;
 #pragma unroll
   for (int i = 0; i < N; i++) masks[i] = p[i];
-}
-
-// One clause of a filter against the tile's table: up to two 15-bit pair
-// hashes (low and high half, 0xFFFF = none), one of which must be present;
-// 0xFFFFFFFF = an unused clause (always true).
-__device__ __forceinline__ bool bg_bigram_clause(const uint32_t *tab, uint32_t k) {
-  if (k == 0xFFFFFFFFu) return true;
-  const uint32_t a = k & 0xFFFFu, b = k >> 16;
-  bool h = (tab[a >> 5] >> (a & 31u)) & 1u;
-  if (b != 0xFFFFu) h = h || ((tab[b >> 5] >> (b & 31u)) & 1u);
-  return h;
 }
 
 // ---- three-input bitwise functions ---------------------------------------------

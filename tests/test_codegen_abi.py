@@ -644,3 +644,33 @@ def test_shared_filter_set_compiles_with_filter_code_object():
     again = compiler.compile_pattern_set([["hello", "ab[0-9]+c"], ["x[yz]q"]],
                                          {"bigram_filter": 1, "shared_filter": 1, "groups_per_kernel": 1})
     assert again.metas[0]["filter"]["hsaco"] == f["hsaco"]
+
+
+def test_bigram_key_words_are_canonical():
+    """Every clause word holds two pair hashes (a one-pair clause repeats its
+    pair) and every filter holds bigram_clauses words (a filter with fewer
+    clauses repeats its first): a clause word is true iff one of its pairs
+    is present, exactly the filter's clause, and ANDing a repeated clause
+    changes nothing -- so the branch-free test gives the filter's answer for
+    every table."""
+    import random as _r
+
+    from bitgen_amd.codegen.gen_hip import GroupProgram
+
+    regs = ["hello", "ab[0-9]+c", "x[yz]q", "q0q0", "[ab]cd", "zz[^\\n]*yy", "a.{0,3}b", "dcba(xy|zq)"]
+    _, _, post, vnm = compiler.group_programs(regs)
+    prog = GroupProgram(0, post, vnm, regs, bigram_filter=True)
+    words = prog.bigram_key_words()
+    ncl = prog.bigram_clauses
+    assert len(words) == ncl * len(prog.bigram_keys)
+    rng = _r.Random(5)
+    for _ in range(200):
+        present = {rng.randrange(1 << 15) for _ in range(rng.randrange(0, 3000))}
+        for f, clauses in enumerate(prog.bigram_keys):
+            # a few of the filter's own pairs planted at random
+            if rng.random() < 0.5:
+                present |= {h for c in clauses for h in c if rng.random() < 0.5}
+            want = all(any(h in present for h in c) for c in clauses)
+            got = all(((w & 0xFFFF) in present) or ((w >> 16) in present)
+                      for w in words[f * ncl:(f + 1) * ncl])
+            assert got == want, (f, clauses)
