@@ -547,6 +547,8 @@ def profiled_counters(args, options):
             found["traffic"] = int(float(line.split("=")[1]))
         elif line.startswith("valu_insts_per_launch"):
             found["valu"] = json.loads(line.split("=", 1)[1])
+        elif line.startswith("salu_insts_per_launch"):
+            found["salu"] = json.loads(line.split("=", 1)[1])
         elif line.startswith("shape ="):
             shape = dict(kv.split("=") for kv in line.split("=", 1)[1].split())
     keys = ("groups_per_kernel", "min_waves", "bigram_filter", "block_threads", "carry_settle", "shared_filter")
@@ -725,6 +727,16 @@ def main():
             "unit": "G wave-instructions/s",
             "frac": round(sum(counters["valu"]) / 1e9 / mean_kernels_s / VALU_PEAK_GINST, 6),
         } if counters and "valu" in counters and len(counters["valu"]) == K else None),
+        # scalar instructions: one scalar unit per CU issues at most one per
+        # cycle (the same 614 G/s); with the byte-pair tables shared, the
+        # group kernels issue more SALU than VALU instructions
+        "roofline_salu": ({
+            "bound": "salu",
+            "achieved": round(sum(counters["salu"]) / 1e9 / mean_kernels_s, 3),
+            "peak": VALU_PEAK_GINST,
+            "unit": "G scalar instructions/s",
+            "frac": round(sum(counters["salu"]) / 1e9 / mean_kernels_s / VALU_PEAK_GINST, 6),
+        } if counters and "salu" in counters and len(counters["salu"]) == K else None),
     }
     if count is not None:
         out["match_count"] = count

@@ -106,29 +106,46 @@ def test_batched_bigram_filter(gpu):
 
 
 
+SHARED_SETS = [
+    [["hello", "ab[0-9]+c"], ["x[yz]q", "w.*d"]],
+    [["abcd", "zz[^\\n]*yy"], ["q0q0", "dcba(xy|zq)"], ["[ab]cd", "ca.*bd"]],
+]
+
+
 def test_batched_shared_filter_eager_and_graph(gpu):
-    """Sets whose kernels share their byte-pair answers: one batched filter
-    pre-pass per set over the batch's global tiles (each mapped to its job's
-    buffer and tile), then the set's kernels read their rows; eager and
-    graph replays (the pre-pass is a kernel node), every job bit-exact."""
+    """Sets whose kernels share their byte-pair answers (two or more kernels
+    with filters): one batched filter pre-pass per set over the batch's
+    global tiles (each mapped to its job's buffer and tile), then the set's
+    kernels read their rows; eager and graph replays (the pre-pass is a
+    kernel node), every job bit-exact."""
     import torch
 
     sets = [BatchPatternSet(s, {"bigram_filter": 1, "shared_filter": 1, "groups_per_kernel": k}).load(0)
-            for s, k in zip(SETS, (1, 2))]
+            for s, k in zip(SHARED_SETS, (1, 2))]
     assert all(s.filter_row > 0 and s.filter_kernel is not None for s in sets)
     buffers = _buffers(9, 14)
     # a partial job list: not every buffer in every set
     jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers)) if (si + b) % 3]
     print(f"buffers={[b.size for b in buffers]}", flush=True)
     plan = BatchPlan(sets, buffers, jobs)
+
+    def check():
+        torch.cuda.synchronize()
+        plan.check_errors()
+        for (si, b) in jobs:
+            got = plan.results(si, b).cpu().numpy()
+            ref = run_oracle(SHARED_SETS[si], buffers[b])
+            n = buffers[b].size
+            for g in range(len(SHARED_SETS[si])):
+                assert np.array_equal(O.mask_to_positions(got[g], n + 1), ref[g]), (si, b, g, n)
+
     plan.enqueue()
-    _check(plan, sets, buffers, jobs)
+    check()
     plan.capture()
     for rep in range(3):
         plan.result.fill_(0)
         plan.run_graph()
-        torch.cuda.synchronize()
-        _check(plan, sets, buffers, jobs)
+        check()
         _ = (plan.bytes[:128].to(torch.int32) * 3).sum().item()  # unrelated eager work
     plan.destroy()
     for s in sets:
