@@ -25,7 +25,8 @@ import numpy as np
 
 from .backend.hip import runtime as rt
 from .codegen.gen_hip import FILTER_BATCH_KERNEL, batch_kernel_name, batch_source, compile_sources, filter_source
-from .compiler import DEFAULT_OPTIONS, kernel_meta, kernel_programs, load_cached_set, store_cached_set, _set_dir
+from .compiler import (DEFAULT_OPTIONS, compile_pattern_set, kernel_meta, kernel_programs, load_cached_set,
+                       store_cached_set, _set_dir)
 
 WAVE_WORDS = 64
 
@@ -44,7 +45,13 @@ def n_tiles(meta, W):
 class BatchPatternSet:
     """The groups of one pattern set compiled into persistent batch kernels
     (one kernel per ``groups_per_kernel`` consecutive groups, merged into one
-    program; separate code objects, compiled in parallel)."""
+    program; separate code objects, compiled in parallel).
+
+    A group whose program needs whole streams between its steps (general
+    while loops, graph breaks: meta kind "stream") has no batch kernel: a
+    BatchPlan runs its plan job by job through the plain form of the set
+    (``plain_set``), eagerly -- each while loop's condition is decided on
+    the host -- so a plan holding such a set is not captured in a graph."""
 
     def __init__(self, groups, options=None, block_threads=256):
         o = dict(DEFAULT_OPTIONS)
@@ -58,17 +65,22 @@ class BatchPatternSet:
         self.kernels = None
         self.filter_module = None
         self.filter_kernel = None
+        self.plain = None       # the plain form (sets with stream-level groups)
+        self.plain_set = None
         d = _set_dir(self.groups, dict(o, form="batch", block=block_threads))
         cached = load_cached_set(d, "batch.json")
         if cached is not None:
             self.metas = cached
-            self.hsacos = [m["hsaco"] for m in cached]
+            self.hsacos = [None if m.get("kind") == "stream" else m["hsaco"] for m in cached]
+            if self.has_stream_groups:
+                self.plain = compile_pattern_set(self.groups, o)
             return
         programs = kernel_programs(self.groups, o)
-        if any(getattr(p, "kernel_name", "").startswith("bg_stream_") for p in programs):
-            raise NotImplementedError("groups that run at stream level (graph breaks, while loops) run in the plain "
-                                      "form only, not batched")
         self.metas = [kernel_meta(p) for p in programs]
+        if self.has_stream_groups:
+            self.plain = compile_pattern_set(self.groups, o)
+        stream_kernels = {k for k, m in enumerate(self.metas) if m.get("kind") == "stream"}
+        programs = [p for k, p in enumerate(programs) if k not in stream_kernels]
         sources = {}
         for p in programs:
             path = os.path.join(d, f"batch_{p.gid}.hip")
@@ -85,9 +97,17 @@ class BatchPatternSet:
                 f.write(src)
             sources[fpath] = src
         hs = compile_sources(sources)
-        self.hsacos = [os.path.abspath(hs[os.path.join(d, f"batch_{p.gid}.hip")]) for p in programs]
-        for m, h in zip(self.metas, self.hsacos):
+        self.hsacos = []
+        for k, m in enumerate(self.metas):
+            if k in stream_kernels:
+                # the group's plan runs through the plain form; its meta keeps
+                # the plain code object (so the cache entry stays checkable)
+                m["hsaco"] = self.plain.metas[k]["hsaco"]
+                self.hsacos.append(None)
+                continue
+            h = os.path.abspath(hs[os.path.join(d, f"batch_{m['gid']}.hip")])
             m["hsaco"] = h
+            self.hsacos.append(h)
             if row:
                 m["filter"] = {"hsaco": os.path.abspath(hs[fpath]), "kernel": FILTER_BATCH_KERNEL, "row": row}
         store_cached_set(d, "batch.json", self.metas)
@@ -103,8 +123,11 @@ class BatchPatternSet:
     def load(self, device=0):
         rt.init(device)
         if self.modules is None:
-            self.modules = [rt.Module(h) for h in self.hsacos]
-            self.kernels = [m.kernel(batch_kernel_name(k)) for k, m in enumerate(self.modules)]
+            self.modules = [rt.Module(h) if h else None for h in self.hsacos]
+            self.kernels = [m.kernel(batch_kernel_name(k)) if m is not None else None
+                            for k, m in enumerate(self.modules)]
+            if self.plain is not None:
+                self.plain_set = self.plain.load(device)
             f = next((m["filter"] for m in self.metas if m.get("filter")), None)
             if f is not None:
                 self.filter_module = rt.Module(f["hsaco"])
@@ -112,19 +135,28 @@ class BatchPatternSet:
         return self
 
     @property
+    def has_stream_groups(self):
+        """True when a group runs its plan at stream level (no batch kernel)."""
+        return any(m.get("kind") == "stream" for m in self.metas)
+
+    @property
     def filter_row(self):
         """Answer words per tile of the set's shared byte-pair filters (0: none)."""
         return max((m.get("filter_row", 0) for m in self.metas), default=0)
 
     def info(self):
-        return [m.info(batch_kernel_name(k)) for k, m in enumerate(self.modules)]
+        return [m.info(batch_kernel_name(k)) if m is not None else None for k, m in enumerate(self.modules)]
 
     def free(self):
         if self.modules is not None:
             for m in self.modules:
-                m.free()
+                if m is not None:
+                    m.free()
             self.modules = None
             self.kernels = None
+        if self.plain_set is not None:
+            self.plain_set.free()
+            self.plain_set = None
         if self.filter_module is not None:
             self.filter_module.free()
             self.filter_module = None
@@ -246,6 +278,7 @@ class BatchPlan:
         self.tdesc = torch.from_numpy(np.frombuffer(bytes(td), dtype=np.uint8).copy()).to(self.dev)
         self.grid_blocks = grid_blocks
         self.graph = None
+        self._stream_status = {}  # status words of stream-level groups, per (set, W)
 
     # -- execution -------------------------------------------------------------
     def _grid(self, ps, total_tiles):
@@ -272,17 +305,50 @@ class BatchPlan:
                 if fq["total_tiles"]:
                     rt.filter_batch_launch(ps.filter_kernel, fq["jobs"].data_ptr(), fq["tile_job"].data_ptr(),
                                            fq["total_tiles"], f["buf"].data_ptr(), stream)
-            if q["total_tiles"]:
+            if ps.metas[q["kernel"]].get("kind") == "stream":
+                self._run_stream_jobs(ps, q)
+            elif q["total_tiles"]:
                 rt.batch_launch(ps.kernels[q["kernel"]], q["args"], self._grid(ps, q["total_tiles"]),
                                 ps.block_threads, stream)
 
+    def _run_stream_jobs(self, ps, q):
+        """A stream-level group over each of its jobs: the group's plan
+        (subgraph kernels, copies, while loops decided on the host) through
+        the set's plain form, on the current torch stream, over the job's
+        basis rows into its result row."""
+        torch = self.torch
+        k = q["kernel"]
+        for e in q["entries"]:
+            W = e["W"]
+            st = self._stream_status.get((id(ps), W))
+            if st is None:
+                st = ps.plain_set.alloc_status(1, W)
+                self._stream_status[(id(ps), W)] = st
+            boff = self.basis_off[e["buffer"]]
+            basis = self.basis[boff:boff + 8 * W].view(8, W)
+            row = self.result[e["res_off"]:e["res_off"] + W]
+            ps.plain_set._run_stream(k, basis, row, st.data_ptr() + k * st.size(2) * 4, W,
+                                     torch.cuda.current_stream(self.dev))
+
+    @property
+    def graph_capturable(self):
+        """False when a set has stream-level groups (their loops are decided
+        on the host between launches)."""
+        return not any(ps.has_stream_groups for ps in self.sets)
+
     def enqueue(self, stream=None):
+        """Transpose + every set's kernels on ``stream`` (a torch stream or
+        None: the current one; sets with stream-level groups run them on the
+        current torch stream, so pass None or make ``stream`` current)."""
         s = rt.current_stream_ptr(stream)
         self.enqueue_transpose(s)
         self.enqueue_match(s)
 
     def capture(self, stream=None):
         """Capture transpose + batch launches into a graph (replay with run_graph)."""
+        if not self.graph_capturable:
+            raise NotImplementedError("a set with stream-level groups (while loops, graph breaks) decides its loops "
+                                      "on the host: run the plan with enqueue(), not as a graph")
         torch = self.torch
         cap = torch.cuda.Stream(device=self.dev)
         cap.wait_stream(torch.cuda.current_stream(self.dev))

@@ -180,6 +180,9 @@ class MatchServer:
         for sid, ps in self.sets.items():
             if ps.kernels is None:
                 raise ValueError(f"pattern set {sid} is not loaded (call BatchPatternSet.load first)")
+            if ps.has_stream_groups:
+                raise ValueError(f"pattern set {sid} has groups that run at stream level (while loops, graph "
+                                 "breaks); the server's native staging runs batch kernels only")
             if ps.filter_row:
                 # the native staging launches the set's kernels only; their
                 # byte-pair tables must be built in the kernels

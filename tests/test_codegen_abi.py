@@ -674,3 +674,20 @@ def test_bigram_key_words_are_canonical():
             got = all(((w & 0xFFFF) in present) or ((w >> 16) in present)
                       for w in words[f * ncl:(f + 1) * ncl])
             assert got == want, (f, clauses)
+
+
+def test_batch_set_with_stream_level_groups_compiles():
+    """A set with a general loop: batch kernels for its tile-parallel groups,
+    none for the stream-level group (its meta keeps the plain form's code
+    object and its plan), the plain form compiled beside it; the second
+    construction comes from the cache."""
+    from bitgen_amd.batch import BatchPatternSet
+
+    groups = [["a(bc)*d", "xy"], ["hello", "w.*d"]]
+    bs = BatchPatternSet(groups)
+    kinds = [m.get("kind") for m in bs.metas]
+    assert kinds[0] == "stream" and kinds[1] != "stream"
+    assert bs.has_stream_groups and bs.hsacos[0] is None and os.path.exists(bs.hsacos[1])
+    assert bs.plain is not None and bs.plain.metas[0].get("kind") == "stream"
+    again = BatchPatternSet(groups)
+    assert again.hsacos == bs.hsacos and again.plain is not None

@@ -176,3 +176,45 @@ def test_batched_phased_graph(gpu, settle):
     plan.destroy()
     for s in sets:
         s.free()
+
+
+LOOP_SETS = [
+    [["a(bc)*d", "xy"], ["hello", "w.*d"]],
+    [["(ab)+c"], ["q0q0"], ["(a|bb)+c", "zz[^\\n]*yy"]],
+]
+
+
+def test_batched_stream_level_groups_eager(gpu):
+    """Sets with groups whose loops are decided on the whole stream: the
+    tile-parallel groups run as batch kernels, the stream-level groups run
+    their plan job by job through the plain form (eagerly); every job
+    bit-exact; such a plan is refused for graph capture."""
+    import torch
+
+    sets = [BatchPatternSet(s).load(0) for s in LOOP_SETS]
+    assert all(s.has_stream_groups for s in sets)
+    buffers = _buffers(13, 10)
+    for b in buffers[:4]:
+        if b.size > 40:
+            b[10:17] = np.frombuffer(b"abcbcbd", dtype=np.uint8)
+            b[25:32] = np.frombuffer(b"abababc", dtype=np.uint8)
+    jobs = [(si, b) for si in range(len(sets)) for b in range(len(buffers)) if (si + b) % 4]
+    print(f"buffers={[b.size for b in buffers]}", flush=True)
+    plan = BatchPlan(sets, buffers, jobs)
+    assert not plan.graph_capturable
+    for rep in range(2):
+        plan.result.fill_(0x5A)
+        plan.enqueue()
+        torch.cuda.synchronize()
+        plan.check_errors()
+        for (si, b) in jobs:
+            got = plan.results(si, b).cpu().numpy()
+            ref = run_oracle(LOOP_SETS[si], buffers[b])
+            n = buffers[b].size
+            for g in range(len(LOOP_SETS[si])):
+                assert np.array_equal(O.mask_to_positions(got[g], n + 1), ref[g]), (rep, si, b, g, n)
+    with pytest.raises(NotImplementedError, match="enqueue"):
+        plan.capture()
+    plan.destroy()
+    for s in sets:
+        s.free()
