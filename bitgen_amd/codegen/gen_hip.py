@@ -198,6 +198,7 @@ class GroupProgram:
             raise HipCodegenError(f"group {gid}: byte-pair filters need the basis streams as inputs")
         self.bigram_keys = []
         self.bigram_clauses = 0
+        self.bigram_pairs = 2
         if bigram_filter:
             self._bigram_rewrite()
         self._check_regions()
@@ -405,6 +406,9 @@ class GroupProgram:
         if not plan:
             return
         self.bigram_clauses = max(len(c) for _, c, _ in plan)
+        # pairs per clause (1: every clause is a single pair; the device
+        # test then reads one table word per clause)
+        self.bigram_pairs = max(len(cl) for _, c, _ in plan for cl in c)
         moved = {j for _, _, f in plan for j in f}
         at = {k: (c, f) for k, c, f in plan}
         out = []
@@ -614,7 +618,7 @@ class GroupProgram:
             add("  (void)bgf;")
             add("  bg_bigram_build(bas, bgt);")
             add(f"  uint32_t bgm[{self.filter_words}];")
-            add(f"  bg_bigram_lookup<{nf}, {self.bigram_clauses}>(bgt, bg_keys_{self.gid}, bgm);")
+            add(f"  bg_bigram_lookup<{nf}, {self.bigram_clauses}, {self.bigram_pairs}>(bgt, bg_keys_{self.gid}, bgm);")
         else:
             add("  (void)bgf;")
         # declarations (goto must not cross initialisations)
@@ -1337,7 +1341,8 @@ def filter_source(programs):
         for c in range((nf + WAVE - 1) // WAVE):
             # every clause evaluated (no short-circuit: no branch per clause)
             test = " & ".join([f"(uint32_t)({c * 64} + lane < {nf})"]
-                              + [f"bg_bigram_clause(bgt, kr{p.gid}[{c}][{j}])" for j in range(ncl)])
+                              + [f"bg_bigram_clause<{p.bigram_pairs}>(bgt, kr{p.gid}[{c}][{j}])"
+                                 for j in range(ncl)])
             # the fence keeps this chunk's table reads from being hoisted
             # ahead of the earlier chunks' (each would hold registers)
             tile += [

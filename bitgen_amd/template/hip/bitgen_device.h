@@ -185,18 +185,22 @@ __device__ __forceinline__ void bg_bigram_build(const uint64_t bas[8], uint32_t 
 // One clause of a filter against the tile's table: two 15-bit pair hashes
 // (low and high half; a one-pair clause repeats its pair, a filter with
 // fewer clauses repeats its first), one of which must be present.  No
-// branch: both table words are read.
+// branch.  NP = 1: every clause of the program has one pair (both halves
+// equal), so only the low half is read.
+template <int NP>
 __device__ __forceinline__ uint32_t bg_bigram_clause(const uint32_t *tab, uint32_t k) {
-  const uint32_t a = k & 0xFFFFu, b = k >> 16;
+  const uint32_t a = k & 0xFFFFu;
+  if (NP == 1) return (tab[a >> 5] >> (a & 31u)) & 1u;
+  const uint32_t b = k >> 16;
   return ((tab[a >> 5] >> (a & 31u)) | (tab[b >> 5] >> (b & 31u))) & 1u;
 }
 
 // Wave-uniform answers of NF filters of NCL clauses each: bit f % 32 of
 // masks[f / 32] is set when every clause of filter f may hold in the tile.
 // keys[f * NCL + c] holds clause c (bg_bigram_clause: two 15-bit pair
-// hashes, one of which must be in the table).  32-bit masks keep each test a
+// hashes, one of which must be in the table; NP pairs per clause).  32-bit masks keep each test a
 // single scalar AND (its result sets SCC for the branch).
-template <int NF, int NCL>
+template <int NF, int NCL, int NP>
 __device__ __forceinline__ void bg_bigram_lookup(const uint32_t *tab, const uint32_t *keys, uint32_t *masks) {
   const int lane = bg_lane();
 #pragma unroll
@@ -205,7 +209,7 @@ __device__ __forceinline__ void bg_bigram_lookup(const uint32_t *tab, const uint
     const int fc = f < NF ? f : NF - 1;  // lanes past the last filter: any key, answer dropped
     uint32_t hit = f < NF;
 #pragma unroll
-    for (int j = 0; j < NCL; j++) hit &= bg_bigram_clause(tab, keys[fc * NCL + j]);
+    for (int j = 0; j < NCL; j++) hit &= bg_bigram_clause<NP>(tab, keys[fc * NCL + j]);
     const uint64_t m = __builtin_amdgcn_ballot_w64(hit != 0u);
     masks[2 * c] = (uint32_t)m;
     masks[2 * c + 1] = (uint32_t)(m >> 32);

@@ -621,7 +621,7 @@ def test_shared_filter_geometry_and_sources():
         assert f"kr{p.gid}[c][j] = c * 64 + lane < {len(p.bigram_keys)} ? bg_fkeys_{p.gid}[" in fsrc
         nchunks = (len(p.bigram_keys) + 63) // 64
         for c in range(nchunks):
-            assert f"bg_bigram_clause(bgt, kr{p.gid}[{c}][0])" in fsrc
+            assert f"bg_bigram_clause<{p.bigram_pairs}>(bgt, kr{p.gid}[{c}][0])" in fsrc
         for j in range(p.filter_off, p.filter_off + p.filter_words):
             assert f"out[{j // 64}] = lane == {j % 64} ?" in fsrc
     # the batch kernels read the batched pre-pass's rows by global tile
@@ -662,6 +662,11 @@ def test_bigram_key_words_are_canonical():
     prog = GroupProgram(0, post, vnm, regs, bigram_filter=True)
     words = prog.bigram_key_words()
     ncl = prog.bigram_clauses
+    # bigram_pairs: the most pairs of any clause; with 1 both halves of every
+    # word are equal, so the device reads only the low half
+    assert prog.bigram_pairs == max(len(c) for k in prog.bigram_keys for c in k)
+    if prog.bigram_pairs == 1:
+        assert all(w & 0xFFFF == w >> 16 for w in words)
     assert len(words) == ncl * len(prog.bigram_keys)
     rng = _r.Random(5)
     for _ in range(200):
