@@ -1313,7 +1313,6 @@ def filter_source(programs):
     progs = [p for p in programs if getattr(p, "shared_filter", False)]
     row = progs[0].filter_row
     HL, HR, S = progs[0].HL, progs[0].HR, progs[0].S
-    nchunk = (row + WAVE - 1) // WAVE
     wpb = FILTER_BLOCK_THREADS // WAVE
     lines = ['#include "bitgen_device.h"']
     for p in progs:
@@ -1332,54 +1331,57 @@ def filter_source(programs):
         "#pragma unroll",
         "    for (int i = 0; i < 8; i++) bas[i] = inb ? bas[i] : 0ull;",
         "    bg_bigram_build(bas, bgt);",
-        f"    uint32_t out[{nchunk}];",
-        "#pragma unroll",
-        f"    for (int c = 0; c < {nchunk}; c++) out[c] = 0u;",
+        f"    uint32_t *__restrict__ frow = filter + (size_t)g * {row}u;",
     ]
     for p in progs:
         nf, ncl = len(p.bigram_keys), p.bigram_clauses
         for c in range((nf + WAVE - 1) // WAVE):
             # every clause evaluated (no short-circuit: no branch per clause)
-            test = " & ".join([f"(uint32_t)({c * 64} + lane < {nf})"]
-                              + [f"bg_bigram_clause<{p.bigram_pairs}>(bgt, kr{p.gid}[{c}][{j}])"
-                                 for j in range(ncl)])
+            if p.bigram_pairs == 1:
+                # a lane's clause register holds its table word's LDS byte
+                # address above its bit index (bg_fkey_slot): one shift for
+                # the address, one for the bit
+                test = " & ".join(f"bg_fkey_test(bg_tables, kr{p.gid}[{c}][{j}])" for j in range(ncl))
+            else:
+                test = " & ".join(f"bg_bigram_clause<{p.bigram_pairs}>(bgt, kr{p.gid}[{c}][{j}])"
+                                  for j in range(ncl))
+            live = min(WAVE, nf - c * WAVE)
+            # lanes past the last filter: their answers dropped from the ballot
+            keep = "" if live == WAVE else f" & 0x{(1 << live) - 1:016X}ull"
             # the fence keeps this chunk's table reads from being hoisted
             # ahead of the earlier chunks' (each would hold registers)
             tile += [
                 "    {",
                 "      " + " ".join(f"bg_fence(kr{p.gid}[{c}][{j}]);" for j in range(ncl)),
-                f"      const uint64_t m = __builtin_amdgcn_ballot_w64(({test}) != 0u);",
+                f"      const uint64_t m = __builtin_amdgcn_ballot_w64((({test}) & 1u) != 0u){keep};",
             ]
-            for half in range(2):
-                j = p.filter_off + 2 * c + half
-                val = "(uint32_t)m" if half == 0 else "(uint32_t)(m >> 32)"
-                tile.append(f"      out[{j // WAVE}] = lane == {j % WAVE} ? {val} : out[{j // WAVE}];")
-            tile.append("    }")
+            # the chunk's two answer words, stored by lane 0 as it goes (no
+            # ballot held across chunks); rows and offsets are even
+            j = p.filter_off + 2 * c
+            tile += [f"      if (lane == 0) *reinterpret_cast<uint64_t *>(frow + {j}) = m;", "    }"]
     keys = ["  // every lane's clauses, once per wave"]
     for p in progs:
         nf, ncl = len(p.bigram_keys), p.bigram_clauses
         nc = (nf + WAVE - 1) // WAVE
+        slot = "bg_fkey_slot(wbase, {})" if p.bigram_pairs == 1 else "{}"
+        key = f"bg_fkeys_{p.gid}[(c * 64 + lane) * {ncl} + j]"
         keys += [
             f"  uint32_t kr{p.gid}[{nc}][{ncl}];",
             "#pragma unroll",
             f"  for (int c = 0; c < {nc}; c++)",
             "#pragma unroll",
             f"    for (int j = 0; j < {ncl}; j++)",
-            f"      kr{p.gid}[c][j] = c * 64 + lane < {nf} ? bg_fkeys_{p.gid}[(c * 64 + lane) * {ncl} + j] : 0u;",
+            f"      kr{p.gid}[c][j] = c * 64 + lane < {nf} ? {slot.format(key)} : 0u;",
         ]
     head = [
         f"  __shared__ uint32_t bg_tables[{wpb}][BG_BIGRAM_WORDS];",
         "  uint32_t *bgt = bg_tables[threadIdx.x / BG_WAVE];",
+        "  const uint32_t wbase = (threadIdx.x / BG_WAVE) * (BG_BIGRAM_WORDS * 4u);",
         f"  constexpr int HL = {HL}, HR = {HR}, S = {S};",
         "  const int lane = bg_lane();",
         *keys,
         f"  const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * {wpb}u + threadIdx.x / BG_WAVE));",
         f"  const uint32_t stride = gridDim.x * {wpb}u;",
-    ]
-    store = [
-        "#pragma unroll",
-        f"    for (int c = 0; c < {nchunk}; c++)",
-        f"      if (c * 64 + lane < {row}) filter[(size_t)g * {row}u + c * 64 + lane] = out[c];",
     ]
     lines += [
         f'extern "C" __global__ __launch_bounds__({FILTER_BLOCK_THREADS}) void {FILTER_KERNEL}('
@@ -1389,7 +1391,6 @@ def filter_source(programs):
         "  for (uint32_t g = first; g < n_tiles; g += stride) {",
         "    const uint32_t t = g;",
         *tile,
-        *store,
         "  }",
         "}",
         "",
@@ -1403,7 +1404,6 @@ def filter_source(programs):
         "    const int64_t W = (int64_t)J.n_words;",
         "    const uint32_t t = g - J.tile_start;",
         *tile,
-        *store,
         "  }",
         "}",
         "",

@@ -15,8 +15,12 @@
 
 #if defined(__HIPCC__)
 #define BG_HD __host__ __device__ __forceinline__
+// fully unrolled: an indexed loop over the register arrays would move its
+// words through relative register addressing
+#define BG_UNROLL8 _Pragma("unroll")
 #else
 #define BG_HD static inline
+#define BG_UNROLL8
 #endif
 
 BG_HD uint64_t bg_bit_transpose8(uint64_t x) {
@@ -40,6 +44,7 @@ BG_HD uint64_t bg_bit_transpose8(uint64_t x) {
 
 BG_HD void bg_transpose_block(const uint64_t x_in[8], uint64_t y[8]) {
   uint64_t x[8];
+  BG_UNROLL8
   for (int j = 0; j < 8; j++) x[j] = bg_bit_transpose8(x_in[j]);
   // byte matrix M[j][b] = byte b of x[j]; want T[b][j].
   BG_SWAP(x[0], x[4], 32, 0x00000000FFFFFFFFull);
@@ -54,6 +59,7 @@ BG_HD void bg_transpose_block(const uint64_t x_in[8], uint64_t y[8]) {
   BG_SWAP(x[2], x[3], 8, 0x00FF00FF00FF00FFull);
   BG_SWAP(x[4], x[5], 8, 0x00FF00FF00FF00FFull);
   BG_SWAP(x[6], x[7], 8, 0x00FF00FF00FF00FFull);
+  BG_UNROLL8
   for (int b = 0; b < 8; b++) y[7 - b] = x[b];
 }
 
@@ -63,6 +69,7 @@ BG_HD void bg_transpose_block(const uint64_t x_in[8], uint64_t y[8]) {
 // stages of bg_transpose_block run again in reverse order.
 BG_HD void bg_untranspose_block(const uint64_t y[8], uint64_t x_out[8]) {
   uint64_t x[8];
+  BG_UNROLL8
   for (int b = 0; b < 8; b++) x[b] = y[7 - b];
   BG_SWAP(x[0], x[1], 8, 0x00FF00FF00FF00FFull);
   BG_SWAP(x[2], x[3], 8, 0x00FF00FF00FF00FFull);
@@ -76,6 +83,7 @@ BG_HD void bg_untranspose_block(const uint64_t y[8], uint64_t x_out[8]) {
   BG_SWAP(x[1], x[5], 32, 0x00000000FFFFFFFFull);
   BG_SWAP(x[2], x[6], 32, 0x00000000FFFFFFFFull);
   BG_SWAP(x[3], x[7], 32, 0x00000000FFFFFFFFull);
+  BG_UNROLL8
   for (int j = 0; j < 8; j++) x_out[j] = bg_bit_transpose8(x[j]);
 }
 

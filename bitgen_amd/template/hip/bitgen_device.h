@@ -195,6 +195,23 @@ __device__ __forceinline__ uint32_t bg_bigram_clause(const uint32_t *tab, uint32
   return ((tab[a >> 5] >> (a & 31u)) | (tab[b >> 5] >> (b & 31u))) & 1u;
 }
 
+// A single-pair clause as a lane of the filter pre-pass holds it for a whole
+// launch: the LDS byte address of its table word (wbase: the byte offset of
+// the wave's table in the block's tables) shifted up by 5, over the bit index
+// in that word (k: the clause's key word; its low half is the pair hash).
+__device__ __forceinline__ uint32_t bg_fkey_slot(uint32_t wbase, uint32_t k) {
+  const uint32_t a = k & 0xFFFFu;
+  return ((wbase + (a >> 5) * 4u) << 5) | (a & 31u);
+}
+
+// The clause's table bit (bit 0 of the result; higher bits are garbage),
+// tabs: the block's tables.
+template <class T>
+__device__ __forceinline__ uint32_t bg_fkey_test(const T &tabs, uint32_t s) {
+  const uint32_t w = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(&tabs[0][0]) + (s >> 5));
+  return w >> (s & 31u);
+}
+
 // Wave-uniform answers of NF filters of NCL clauses each: bit f % 32 of
 // masks[f / 32] is set when every clause of filter f may hold in the tile.
 // keys[f * NCL + c] holds clause c (bg_bigram_clause: two 15-bit pair

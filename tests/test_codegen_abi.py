@@ -618,12 +618,17 @@ def test_shared_filter_geometry_and_sources():
     for p in shared:
         # lane f holds filter f's clauses; one ballot per 64 filters, two
         # answer words per ballot at the kernel's offset
-        assert f"kr{p.gid}[c][j] = c * 64 + lane < {len(p.bigram_keys)} ? bg_fkeys_{p.gid}[" in fsrc
         nchunks = (len(p.bigram_keys) + 63) // 64
         for c in range(nchunks):
-            assert f"bg_bigram_clause<{p.bigram_pairs}>(bgt, kr{p.gid}[{c}][0])" in fsrc
-        for j in range(p.filter_off, p.filter_off + p.filter_words):
-            assert f"out[{j // 64}] = lane == {j % 64} ?" in fsrc
+            if p.bigram_pairs == 1:
+                # single-pair clauses: lanes hold table slots, not hashes
+                assert f"bg_fkey_test(bg_tables, kr{p.gid}[{c}][0])" in fsrc
+            else:
+                assert f"bg_bigram_clause<{p.bigram_pairs}>(bgt, kr{p.gid}[{c}][0])" in fsrc
+            # lane 0 stores the chunk's two answer words at the kernel's offset
+            assert f"if (lane == 0) *reinterpret_cast<uint64_t *>(frow + {p.filter_off + 2 * c}) = m;" in fsrc
+        slot = "bg_fkey_slot(wbase, bg_fkeys_" if p.bigram_pairs == 1 else "bg_fkeys_"
+        assert f"kr{p.gid}[c][j] = c * 64 + lane < {len(p.bigram_keys)} ? {slot}{p.gid}[" in fsrc
     # the batch kernels read the batched pre-pass's rows by global tile
     from bitgen_amd.codegen.gen_hip import batch_source
 
