@@ -610,6 +610,9 @@ def test_shared_filter_geometry_and_sources():
         assert f"void {b.kernel_name}_shared_filter()" in src and "_shared_filter()" not in a.source()
         assert "bg_bigram_build" in a.source() and "bg_filter_words<" not in a.source()
     assert all(p.filter_row == off for p in shared)
+    # lane 0 stores a chunk's two answer words as one 8-byte store: rows and
+    # every kernel's offset are even
+    assert off % 2 == 0 and all(p.filter_off % 2 == 0 for p in shared)
     fsrc = filter_source(shared)
     # one table build per tile in each of the two pre-pass kernels (one
     # buffer, a batch's global tiles)
