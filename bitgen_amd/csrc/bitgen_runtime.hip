@@ -73,20 +73,48 @@ uint32_t *g_scratch[BG_MAX_DEVICES] = {};
 
 // ---------------------------------------------------------------------------
 // Transpose: thread = one 64-byte block (one word of each basis stream) of one
-// input.  Full blocks are read as four 16-byte loads; the tail block is read
+// input.  A wave whose 64 blocks (4 KiB) all hold data reads them with four
+// coalesced 16-byte loads per lane (1 KiB per instruction) and hands each
+// lane its own block through the wave's LDS area (rows padded to 80 bytes);
+// other full blocks are read as four 16-byte loads per lane, the tail block
 // byte by byte with bounds checks; blocks past the data write zero words.
 __global__ __launch_bounds__(256) void bg_transpose_kernel(const uint8_t *__restrict__ bytes,
                                                            uint64_t n_bytes, uint64_t in_stride,
                                                            uint64_t *__restrict__ basis,
                                                            uint64_t n_words) {
+  __shared__ uint4 stage[256 / 64][64 * 5];
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t input = blockIdx.y;
-  if (w >= n_words) return;
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
   const uint8_t *src = bytes + (uint64_t)input * in_stride;
   uint64_t *dst = basis + (uint64_t)input * 8 * n_words;
   const uint64_t off = w * 64;
+  const uint64_t wave_off = (w - lane) * 64;  // the wave's first byte (wave-uniform)
   uint64_t x[8], y[8];
-  if (off + 64 <= n_bytes) {
+  if (wave_off + 4096 <= n_bytes) {
+    // every lane's block is in range (and so w < n_words)
+    const uint4 *p = reinterpret_cast<const uint4 *>(src + wave_off);
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = p[k * 64 + lane];
+    // 16-byte piece e = k * 64 + lane belongs to lane e / 4, part e % 4
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t e = k * 64 + lane;
+      stage[wv][(e >> 2) * 5 + (e & 3)] = v[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint4 r = stage[wv][lane * 5 + k];
+      x[2 * k] = ((uint64_t)r.y << 32) | r.x;
+      x[2 * k + 1] = ((uint64_t)r.w << 32) | r.z;
+    }
+  } else if (w >= n_words) {
+    return;
+  } else if (off + 64 <= n_bytes) {
     const uint4 *p = reinterpret_cast<const uint4 *>(src + off);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
